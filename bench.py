@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: incoherent closest-hit throughput on MI355X.
+
+Workload (one GPU): a displaced icosphere subdivided 8 times (1,310,720 FP32
+triangles, radial value-noise displacement of +/-5 % of the radius) queried with
+2^22 rays — 2^21 coherent pinhole rays on a 2048x1024 grid aimed at the sphere
+and 2^21 incoherent rays (origins uniform in the bounding box, directions
+uniform on the sphere).  One step = one closest-hit launch over the 2^21
+incoherent rays, inputs resident in HBM; outputs are hit flag, (mesh, face),
+t and barycentrics per ray.
+
+Prints one JSON line: metric/value (incoherent Mrays/s), the roofline of the
+traversal kernel (algorithmic bytes per ray from the reference's own per-ray
+counters, which the kernel reproduces exactly), the coherent rate, and the CPU
+baseline (the reference CPU intersector when its build is present, otherwise
+the oracle restatement) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--subdiv", type=int, default=8)
+    ap.add_argument("--rays-log2", type=int, default=22)
+    ap.add_argument("--cpu-sample-log2", type=int, default=17)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(counters: np.ndarray) -> np.ndarray:
+    """B_ray = 24 (ray) + 20 (hit) + 32 * N_box + 36 * N_tri, with N_box the nodes the
+    reference pops and box-tests (num_bb_hits + num_bb_misses) and N_tri its face tests."""
+    n_box = counters["bb_hits"].astype(np.float64) + counters["bb_misses"]
+    return 24.0 + 20.0 + 32.0 * n_box + 36.0 * counters["tri_isects"].astype(np.float64)
+
+
+def cpu_baseline(meshes, rays: np.ndarray, threads: int):
+    import oracle as O
+    from polytope_amd import scenes as S
+    if O.reference_available():
+        with tempfile.TemporaryDirectory() as d:
+            S.write_ptxm(f"{d}/s.bin", meshes)
+            S.write_rays(f"{d}/r.bin", rays)
+            p = subprocess.run([str(O.REF_DRIVER), "--ptxm", f"{d}/s.bin", "--time-rays", f"{d}/r.bin",
+                                "--threads", str(threads)], capture_output=True, text=True, timeout=900, cwd=d)
+            line = [ln for ln in p.stdout.splitlines() if ln.startswith("{\"rays\"")]
+            if p.returncode == 0 and line:
+                r = json.loads(line[-1])
+                return {"value": round(r["mrays_per_s"], 4), "unit": "Mrays/s", "cores": threads,
+                        "kind": "reference",
+                        "sample": f"first {len(rays)} incoherent rays of the workload, reference "
+                                  f"bvh::intersect_compact, {threads} threads over contiguous ray chunks"}
+    sc = O.OracleScene(meshes)
+    sc.intersect_ids(rays[: min(len(rays), 4096)], threads)
+    t0 = time.perf_counter()
+    sc.intersect_ids(rays, threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(rays) / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {len(rays)} incoherent rays of the workload, oracle restatement, {threads} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus_requested": args.gpus}))
+        sys.exit(2)
+    import torch
+    from polytope_amd import COUNTER_DTYPE, Device, Scene
+    from polytope_amd import scenes as S
+
+    n_rays = 1 << args.rays_log2
+    t0 = time.perf_counter()
+    meshes, rays, n_coh = S.closest_hit_workload(args.subdiv, n_rays, seed=2)
+    print(f"[bench] workload built: {meshes[0].num_faces} triangles, {n_rays} rays "
+          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    dev = Device(0)
+    sc = Scene(dev, meshes)
+    info = sc.info()
+    print(f"[bench] scene: {info['num_nodes']} nodes, build {info['build_seconds']:.2f} s, "
+          f"{info['device_bytes'] / 1e6:.1f} MB on device", file=sys.stderr, flush=True)
+
+    d_rays = torch.from_numpy(rays).cuda()
+    n_inc = n_rays - n_coh
+    inc_ptr = d_rays.data_ptr() + n_coh * 24
+    hits = torch.empty(n_rays * 6, dtype=torch.int32, device="cuda")
+    cnt = torch.empty(n_rays * 3, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    # algorithmic bytes of the incoherent half (untimed run with counters)
+    sc.intersect_device(inc_ptr, n_inc, hits.data_ptr(), cnt.data_ptr(), 0, stream.cuda_stream)
+    torch.cuda.synchronize()
+    c = cnt[: n_inc * 3].cpu().numpy().view(COUNTER_DTYPE)
+    b_ray = algorithmic_bytes(c)
+    mean_b = float(b_ray.mean())
+    # coherent rate (untimed by the step loop)
+    for _ in range(2):
+        sc.intersect_device(d_rays.data_ptr(), n_coh, hits.data_ptr(), 0, 0, stream.cuda_stream)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    for _ in range(5):
+        sc.intersect_device(d_rays.data_ptr(), n_coh, hits.data_ptr(), 0, 0, stream.cuda_stream)
+    torch.cuda.synchronize()
+    coherent_mrays = 5 * n_coh / (time.perf_counter() - tc) / 1e6
+
+    for _ in range(args.warmup):
+        sc.intersect_device(inc_ptr, n_inc, hits.data_ptr(), 0, 0, stream.cuda_stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t_start = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        sc.intersect_device(inc_ptr, n_inc, hits.data_ptr(), 0, 0, stream.cuda_stream)
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    launch_ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    ms_per_step = wall * 1e3 / args.steps
+    mrays = n_inc / (ms_per_step / 1e3) / 1e6
+    achieved_gbs = mean_b * n_inc / (launch_ms / 1e3) / 1e9
+
+    traffic = None
+    tf = ROOT / "profiles" / "closest_hit_traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.skip_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = rays[n_coh: n_coh + (1 << args.cpu_sample_log2)]
+        print(f"[bench] cpu baseline on {len(sample)} rays, {threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(meshes, sample, threads)
+
+    out = {
+        "metric": "incoherent closest-hit Mrays/s @1.3M tris",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded displaced icosphere, seeded rays)",
+        "config": {"workload": "closest-hit microbench: icosphere subdiv 8 (1,310,720 tris), 2^21 incoherent "
+                               "rays per step (of 2^22: half coherent 2048x1024, half incoherent)",
+                   "triangles": meshes[0].num_faces, "rays_per_step": n_inc, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_ray": round(mean_b, 1), "kernel": "k_closest_hit",
+                     "launch_ms": round(launch_ms, 4)},
+        "coherent_mrays_per_s": round(coherent_mrays, 2),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    sc.close()
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

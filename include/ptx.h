@@ -1,0 +1,216 @@
+/*
+ * ptx.h — C ABI of libptx_hip.so, the MI355X (gfx950) path-tracing hot path
+ * for polytope scenes: SAH-BVH closest-hit traversal, Wald watertight
+ * ray/triangle intersection and the Lambertian/mirror bounce loop.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (PTX_OK == 0) with a per-thread message from ptx_last_error().  Nothing in
+ * the library calls exit(): a failed call returns an error and leaves the
+ * process running.  Scene handles are immutable after creation and may be used
+ * from several host threads at once; the caller owns every buffer it passes.
+ *
+ * Reference interfaces each entry replaces (danielthompson/polytope):
+ *   ptx_device_open/close     poly::GPUMemoryManager(w, h) / ~GPUMemoryManager
+ *                             (src/cuda/gpu_memory_manager.h:79-119, .cpp:279-286)
+ *   ptx_scene_create          GPUMemoryManager::MallocScene(Scene*) (gpu_memory_manager.cpp:19-277),
+ *                             fed by bvh::bound + bvh::compact (src/cpu/acceleration/bvh.cpp:36, 695)
+ *   ptx_intersect_closest*    Scene::intersect(Ray&, x, y) -> bvh::intersect_compact
+ *                             (src/cpu/scenes/Scene.cpp:12-31, src/cpu/acceleration/bvh.cpp:476-523),
+ *                             batched over many rays
+ *   ptx_render                PathTracerKernel::Trace(spp) + OutputPNG::Output's device->host
+ *                             film copy (src/cuda/kernels/path_tracer.cu:899-1018,
+ *                             src/cuda/png_output.cpp:12-24), with the CPU path's semantics
+ *                             (PathTraceIntegrator::GetSample, BoxFilter), not the CUDA kernel's
+ *   ptx_render_batch          no counterpart: many render jobs merged into one persistent launch
+ *   ptx_server_*              no counterpart: continuous-batching render service
+ */
+#ifndef PTX_H
+#define PTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+   PTX_OK = 0,
+   PTX_ERR_INVALID = 1,      /* bad argument or scene description            */
+   PTX_ERR_HIP = 2,          /* HIP runtime error                             */
+   PTX_ERR_OOM = 3,          /* device or host allocation failed              */
+   PTX_ERR_BUDGET = 4,       /* job larger than the server's HBM budget       */
+   PTX_ERR_NOT_FOUND = 5,    /* unknown job id                                */
+   PTX_ERR_CANCELLED = 6,    /* job was cancelled                             */
+   PTX_ERR_NO_DEVICE = 7     /* no usable gfx950 device                       */
+};
+
+enum { PTX_BRDF_LAMBERT = 1, PTX_BRDF_MIRROR = 2 };
+
+typedef struct ptx_device ptx_device;
+typedef struct ptx_scene ptx_scene;
+typedef struct ptx_server ptx_server;
+
+/* One mesh instance: poly::mesh_geometry (packed SoA vertices + face index
+ * arrays, src/cpu/shapes/mesh.h:17-72) plus poly::Mesh's transform and
+ * material (mesh.h:77-113).  Matrices are row-major 4x4, as Matrix4x4. */
+typedef struct ptx_mesh_desc {
+   uint32_t num_vertices, num_faces;
+   const float *x, *y, *z;               /* packed vertex positions             */
+   const float *nx, *ny, *nz;            /* optional vertex normals (all NULL)  */
+   const uint32_t *fv0, *fv1, *fv2;      /* per-face vertex indices             */
+   float object_to_world[16];
+   float world_to_object[16];            /* Transform::Inverse, used for normals */
+   int32_t brdf;                         /* PTX_BRDF_LAMBERT or PTX_BRDF_MIRROR  */
+   float refl[3];                        /* Kd (Lambert) or Kr (mirror)          */
+   int32_t is_light;                     /* Mesh::spd != nullptr                 */
+   float emission[3];                    /* *Mesh::spd, 0-255 units              */
+} ptx_mesh_desc;
+
+/* poly::compact_bvh_node, byte-compatible (src/cpu/acceleration/bvh.h:24-53). */
+typedef struct ptx_bvh_node {
+   float p0[3], p1[3];
+   uint32_t offset;        /* interior: high_child_offset; leaf: face_index_offset */
+   uint16_t num_faces;     /* 0 = interior                                         */
+   uint16_t flags;         /* split axis                                           */
+} ptx_bvh_node;
+
+typedef struct ptx_scene_desc {
+   const ptx_mesh_desc *meshes;
+   uint32_t num_meshes;
+   /* Optional prebuilt reference BVH (compact nodes + leaf (mesh, face) pairs,
+    * 2 x u32 each).  NULL: the library builds it on the host with a builder
+    * that takes the same decisions as bvh::bound + bvh::compact. */
+   const ptx_bvh_node *nodes;
+   uint32_t num_nodes;
+   const uint32_t *leaf_pairs;
+   uint32_t num_pairs;
+   int32_t has_skybox;     /* Scene::Skybox != nullptr (ColorSkybox)  */
+   float skybox[3];
+   uint32_t build_threads; /* 0 = all host cores */
+} ptx_scene_desc;
+
+typedef struct ptx_scene_info {
+   uint32_t num_nodes, num_pairs, num_interior, num_leaves, max_depth;
+   uint64_t device_bytes;
+   double build_seconds;   /* host BVH build + layout, 0 for a prebuilt BVH */
+   double upload_seconds;
+} ptx_scene_info;
+
+/* PerspectiveCamera (src/cpu/cameras/PerspectiveCamera.cpp:17-49) with the
+ * constructor's derived values precomputed by the caller on the host. */
+typedef struct ptx_camera_desc {
+   float camera_to_world[16];
+   float fov;
+   uint32_t width, height;
+   float one_over_width, one_over_height, aspect_ratio, tan_fov_over_2;
+} ptx_camera_desc;
+
+/* One render: CenterSampler, PathTraceIntegrator{MaxDepth}, BoxFilter film.
+ * Lambert bounces draw u0, u1 from Philox4x32-10 with key (seed, 0x706F6C79)
+ * and counter (y*W + x, sample, bounce, 0): u = (r >> 8) * 2^-24. */
+typedef struct ptx_job_desc {
+   ptx_camera_desc camera;
+   uint32_t spp, max_depth, seed;
+} ptx_job_desc;
+
+/* Closest hit of one ray.  face == 0xFFFFFFFF: miss (mesh also 0xFFFFFFFF). */
+typedef struct ptx_hit {
+   uint32_t mesh, face;
+   float t, u, v, w;
+} ptx_hit;
+
+/* Shading frame the reference leaves in poly::Intersection after traversal
+ * (src/cpu/shapes/mesh.cpp:366-461): offset Location, flipped normal,
+ * tangents and the error bound. */
+typedef struct ptx_hit_frame {
+   float location[3], normal[3], tangent1[3], tangent2[3], error[3];
+} ptx_hit_frame;
+
+/* The reference's per-ray counters (Intersection::num_bb_hits, num_bb_misses,
+ * num_triangle_isects; bvh.cpp:495, 502, 520). */
+typedef struct ptx_ray_counters {
+   uint32_t bb_hits, bb_misses, tri_isects;
+} ptx_ray_counters;
+
+const char *ptx_last_error(void);
+const char *ptx_version(void);
+
+int ptx_device_open(int ordinal, ptx_device **out);
+void ptx_device_close(ptx_device *dev);
+
+int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **out);
+void ptx_scene_destroy(ptx_scene *scene);
+int ptx_scene_get_info(const ptx_scene *scene, ptx_scene_info *out);
+/* Copies the reference-layout BVH the scene was built from (num_nodes nodes,
+ * 2*num_pairs u32). Either pointer may be NULL. */
+int ptx_scene_copy_bvh(const ptx_scene *scene, ptx_bvh_node *nodes, uint32_t *leaf_pairs);
+
+/* Host-only BVH build, no device needed: the decisions of bvh::bound + bvh::compact
+ * (src/cpu/acceleration/bvh.cpp:36-395, 652-698) over the meshes' world-space
+ * faces.  Query sizes with nodes/leaf_pairs NULL, then call again with buffers
+ * of *num_nodes nodes and 2 * *num_pairs u32.  threads 0 = all host cores. */
+int ptx_bvh_build(const ptx_mesh_desc *meshes, uint32_t num_meshes, uint32_t threads,
+                  ptx_bvh_node *nodes, uint32_t *num_nodes, uint32_t *leaf_pairs, uint32_t *num_pairs);
+
+/* Closest hit for n rays (6 floats each: origin, direction; MinT = +inf),
+ * host buffers.  counters may be NULL. */
+int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_hit *hits,
+                          ptx_ray_counters *counters);
+/* Same, device buffers already resident in HBM, on `stream` (hipStream_t; NULL =
+ * the null stream).  counters and frames may be NULL.  Asynchronous. */
+int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t n,
+                                 ptx_hit *d_hits, ptx_ray_counters *d_counters,
+                                 ptx_hit_frame *d_frames, void *stream);
+
+/* Plain GPU render of one job (one launch series per bounce, as the reference
+ * CUDA path), film_rgb = W*H*3 floats on the host (BoxFilter::Output values). */
+int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb);
+/* Batched render: all jobs' paths merged into shared wavefront queues; each
+ * bounce is one captured hipGraph replay.  films[i] = W_i*H_i*3 host floats. */
+int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_t n_jobs,
+                     float *const *films);
+
+/* Tuning knobs shared by ptx_render_batch and the server; zero = autotuned. */
+typedef struct ptx_tuning {
+   uint32_t paths_per_wave;  /* path slots in the wavefront queues         */
+   uint32_t block_size;      /* threads per block of the traversal kernel  */
+   uint32_t lds_stack;       /* LDS stack entries per lane                 */
+} ptx_tuning;
+int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
+int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);
+/* Times candidate tunings on `scene` and keeps the fastest. */
+int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chosen);
+
+/* ---------------------------------------------------------------- serving */
+enum { PTX_JOB_QUEUED = 0, PTX_JOB_RUNNING = 1, PTX_JOB_DONE = 2, PTX_JOB_FAILED = 3,
+       PTX_JOB_CANCELLED = 4 };
+
+typedef struct ptx_server_config {
+   uint64_t hbm_budget_bytes;    /* films + path state of admitted jobs      */
+   uint32_t max_jobs_in_flight;  /* 0 = unlimited (budget only)              */
+   uint32_t paths_per_wave;      /* 0 = autotuned                            */
+} ptx_server_config;
+
+typedef struct ptx_job_status {
+   int32_t state;                /* PTX_JOB_*                                */
+   int32_t error;                /* PTX_* error for FAILED                   */
+   double submit_time, start_time, end_time;  /* seconds, server clock       */
+   uint64_t samples_done, samples_total;
+} ptx_job_status;
+
+int ptx_server_create(ptx_device *dev, const ptx_server_config *cfg, ptx_server **out);
+void ptx_server_destroy(ptx_server *srv);
+/* Queues a job; scene must outlive it.  Rejects with PTX_ERR_BUDGET a job that
+ * could never fit the budget. */
+int ptx_server_submit(ptx_server *srv, ptx_scene *scene, const ptx_job_desc *job, uint64_t *job_id);
+int ptx_server_status(ptx_server *srv, uint64_t job_id, ptx_job_status *out);
+/* Blocks until the job ends; copies its film (W*H*3 floats) if film_rgb != NULL
+ * and releases the job. */
+int ptx_server_wait(ptx_server *srv, uint64_t job_id, float *film_rgb, ptx_job_status *out);
+int ptx_server_cancel(ptx_server *srv, uint64_t job_id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTX_H */

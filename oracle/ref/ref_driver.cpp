@@ -1,0 +1,328 @@
+// Test-infrastructure driver for the reference polytope CPU path.
+//
+// This file is part of the checker, not of the product: it links the
+// reference's own sources (compiled in place from the reference tree by
+// oracle/ref/Makefile) and exposes them through a small command-line tool that
+// reads scenes and rays from files and writes the reference's outputs
+// (compact BVH nodes, leaf pairs, per-ray hits with counters, rendered film)
+// as raw little-endian binaries.  Those outputs become the golden fixtures in
+// tests/golden, and on a machine where the tool is built it also serves as the
+// "reference" CPU baseline.
+//
+// It defines the globals that the reference's main.cpp defines
+// (src/cpu/main.cpp:27, 50-51) and otherwise only calls reference APIs:
+//   bvh::bound / compact / intersect_compact   (src/cpu/acceleration/bvh.cpp:36, 695, 476)
+//   pbrt_parser::parse_file                     (src/common/parsers/pbrt_parser.cpp:275)
+//   AbstractRunner::Trace, BoxFilter::Output    (src/cpu/runners/AbstractRunner.cpp:11,
+//                                                src/cpu/filters/BoxFilter.cpp:54)
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+#include <atomic>
+
+#include "common/utilities/Common.h"
+#include "common/parsers/pbrt_parser.h"
+#include "cpu/acceleration/bvh.h"
+#include "cpu/cameras/PerspectiveCamera.h"
+#include "cpu/filters/BoxFilter.h"
+#include "cpu/films/PNGFilm.h"
+#include "cpu/integrators/PathTraceIntegrator.h"
+#include "cpu/runners/TileRunner.h"
+#include "cpu/samplers/samplers.h"
+#include "cpu/scenes/Scene.h"
+#include "cpu/scenes/skyboxes/ColorSkybox.h"
+#include "cpu/shading/brdf/lambert_brdf.h"
+#include "cpu/shading/brdf/mirror_brdf.h"
+#include "cpu/shapes/mesh.h"
+#include "cpu/structures/stats.h"
+
+poly::Logger Log;
+struct poly::stats main_stats;
+thread_local struct poly::stats thread_stats;
+
+namespace {
+
+struct Reader {
+   FILE* f;
+   explicit Reader(const char* path) : f(std::fopen(path, "rb")) {
+      if (!f) { std::fprintf(stderr, "cannot open %s\n", path); std::exit(2); }
+   }
+   ~Reader() { if (f) std::fclose(f); }
+   void raw(void* p, size_t n) {
+      if (n && std::fread(p, 1, n, f) != n) { std::fprintf(stderr, "short read\n"); std::exit(2); }
+   }
+   uint32_t u32() { uint32_t v; raw(&v, 4); return v; }
+   float f32() { float v; raw(&v, 4); return v; }
+};
+
+void write_file(const std::string& path, const void* p, size_t n) {
+   FILE* f = std::fopen(path.c_str(), "wb");
+   if (!f) { std::fprintf(stderr, "cannot write %s\n", path.c_str()); std::exit(2); }
+   if (n) std::fwrite(p, 1, n, f);
+   std::fclose(f);
+}
+
+struct Job { unsigned width = 0, height = 0, spp = 1, maxdepth = 5; };
+
+// Scene container in the PTXM binary format written by polytope_amd/scenes.py:
+//   u32 'PTXM', u32 version(1), u32 n_meshes
+//   per mesh: u32 nv, u32 nf, u32 has_normals, f32 object_to_world[16] (row-major),
+//             u32 brdf (1 lambert, 2 mirror), f32 refl[3], u32 is_light, f32 L[3],
+//             f32 x[nv], y[nv], z[nv], [nx,ny,nz], u32 fv0[nf], fv1[nf], fv2[nf]
+//   u32 has_camera, f32 camera_to_world[16], f32 fov, u32 W, u32 H, u32 spp, u32 maxdepth
+//   u32 has_skybox, f32 sky[3]
+std::unique_ptr<poly::AbstractRunner> load_ptxm(const char* path, Job& job) {
+   Reader r(path);
+   if (r.u32() != 0x4D585450u) { std::fprintf(stderr, "bad magic\n"); std::exit(2); }
+   if (r.u32() != 1u) { std::fprintf(stderr, "bad version\n"); std::exit(2); }
+   const uint32_t n_meshes = r.u32();
+   struct PendingMesh {
+      std::shared_ptr<poly::Transform> o2w, w2o;
+      std::shared_ptr<poly::Material> mat;
+      std::shared_ptr<poly::mesh_geometry> geo;
+      bool light; float L[3];
+   };
+   std::vector<PendingMesh> pending;
+   for (uint32_t m = 0; m < n_meshes; m++) {
+      const uint32_t nv = r.u32(), nf = r.u32(), has_n = r.u32();
+      float mat[4][4];
+      r.raw(mat, sizeof(mat));
+      const uint32_t brdf = r.u32();
+      float refl[3]; r.raw(refl, sizeof(refl));
+      const uint32_t light = r.u32();
+      float L[3]; r.raw(L, sizeof(L));
+      std::vector<float> x(nv), y(nv), z(nv), nx, ny, nz;
+      r.raw(x.data(), 4 * nv); r.raw(y.data(), 4 * nv); r.raw(z.data(), 4 * nv);
+      if (has_n) {
+         nx.resize(nv); ny.resize(nv); nz.resize(nv);
+         r.raw(nx.data(), 4 * nv); r.raw(ny.data(), 4 * nv); r.raw(nz.data(), 4 * nv);
+      }
+      std::vector<uint32_t> f0(nf), f1(nf), f2(nf);
+      r.raw(f0.data(), 4 * nf); r.raw(f1.data(), 4 * nf); r.raw(f2.data(), 4 * nf);
+
+      auto geo = std::make_shared<poly::mesh_geometry>();
+      for (uint32_t i = 0; i < nv; i++) {
+         poly::Point p(x[i], y[i], z[i]);
+         if (has_n) {
+            poly::Normal n(nx[i], ny[i], nz[i]);
+            geo->add_vertex(p, n);
+         } else {
+            geo->add_vertex(p);
+         }
+      }
+      geo->has_vertex_normals = has_n != 0;
+      for (uint32_t i = 0; i < nf; i++) geo->add_packed_face(f0[i], f1[i], f2[i]);
+      geo->unpack_faces();
+
+      auto o2w = std::make_shared<poly::Transform>(mat);
+      auto w2o = std::make_shared<poly::Transform>(o2w->Invert());
+      std::shared_ptr<poly::AbstractBRDF> b;
+      if (brdf == 2)
+         b = std::make_shared<poly::MirrorBRDF>(poly::ReflectanceSpectrum(refl[0], refl[1], refl[2]));
+      else
+         b = std::make_shared<poly::LambertBRDF>(poly::ReflectanceSpectrum(refl[0], refl[1], refl[2]));
+      auto material = std::make_shared<poly::Material>(b);
+      pending.push_back({o2w, w2o, material, geo, light != 0, {L[0], L[1], L[2]}});
+   }
+   const uint32_t has_cam = r.u32();
+   float cam[4][4]; r.raw(cam, sizeof(cam));
+   const float fov = r.f32();
+   job.width = r.u32(); job.height = r.u32(); job.spp = r.u32(); job.maxdepth = r.u32();
+   const uint32_t has_sky = r.u32();
+   float sky[3]; r.raw(sky, sizeof(sky));
+   if (!has_cam) { job.width = job.height = 1; }
+
+   const poly::Bounds bounds(job.width, job.height);
+   const poly::CameraSettings settings(bounds, fov);
+   std::unique_ptr<poly::AbstractCamera> camera =
+         std::make_unique<poly::PerspectiveCamera>(settings, poly::Transform(cam), true);
+   auto* scene = new poly::Scene(std::move(camera));
+   scene->bvh_root.num_nodes = 0;
+   for (auto& p : pending) {
+      auto* mesh = new poly::Mesh(p.o2w, p.w2o, p.mat, p.geo);
+      if (p.light) {
+         mesh->spd = std::make_shared<poly::SpectralPowerDistribution>(p.L[0], p.L[1], p.L[2]);
+         scene->Lights.push_back(mesh);
+      }
+      scene->Shapes.push_back(mesh);
+   }
+   if (has_sky)
+      scene->Skybox = std::make_unique<poly::ColorSkybox>(poly::SpectralPowerDistribution(sky[0], sky[1], sky[2]));
+
+   auto filter = std::make_unique<poly::BoxFilter>(bounds);
+   filter->SetSamples(job.spp);
+   auto film = std::make_unique<poly::PNGFilm>(bounds, "unused.png", std::move(filter));
+   auto integrator = std::make_unique<poly::PathTraceIntegrator>(scene, (int)job.maxdepth);
+   auto sampler = std::make_unique<poly::CenterSampler>();
+   return std::make_unique<poly::TileRunner>(std::move(sampler), scene, std::move(integrator),
+                                             std::move(film), bounds, job.spp);
+}
+
+// Per-ray record written by "hits": 25 four-byte words.
+struct HitRecord {
+   uint32_t hit, mesh, face;
+   float t, u, v, w;
+   float loc[3], geo_n[3], t1[3], t2[3], err[3];
+   uint32_t bb_hits, bb_misses, tri_isects;
+};
+static_assert(sizeof(HitRecord) == 100, "hit record layout");
+
+void usage() {
+   std::fprintf(stderr,
+      "usage: ref_driver (--ptxm scene.bin | --pbrt scene.pbrt) [--nodes out] [--pairs out]\n"
+      "                  [--rays rays.bin --hits out] [--render out] [--threads N]\n"
+      "                  [--time-rays rays.bin --threads N --repeat R]\n");
+   std::exit(2);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+   std::string ptxm, pbrt, nodes_out, pairs_out, rays_in, hits_out, render_out, time_rays;
+   unsigned threads = 1, repeat = 1;
+   for (int i = 1; i < argc; i++) {
+      std::string a = argv[i];
+      auto next = [&]() -> std::string { if (i + 1 >= argc) usage(); return argv[++i]; };
+      if (a == "--ptxm") ptxm = next();
+      else if (a == "--pbrt") pbrt = next();
+      else if (a == "--nodes") nodes_out = next();
+      else if (a == "--pairs") pairs_out = next();
+      else if (a == "--rays") rays_in = next();
+      else if (a == "--hits") hits_out = next();
+      else if (a == "--render") render_out = next();
+      else if (a == "--time-rays") time_rays = next();
+      else if (a == "--threads") threads = (unsigned)std::atoi(next().c_str());
+      else if (a == "--repeat") repeat = (unsigned)std::atoi(next().c_str());
+      else usage();
+   }
+   Job job;
+   std::unique_ptr<poly::AbstractRunner> runner;
+   if (!ptxm.empty()) {
+      runner = load_ptxm(ptxm.c_str(), job);
+   } else if (!pbrt.empty()) {
+      poly::pbrt_parser parser;
+      runner = parser.parse_file(pbrt);
+      runner->Scene->bvh_root.num_nodes = 0;
+      job.width = runner->Bounds.x; job.height = runner->Bounds.y; job.spp = runner->NumSamples;
+   } else {
+      usage();
+   }
+   poly::Scene* scene = runner->Scene;
+
+   const auto t0 = std::chrono::steady_clock::now();
+   scene->bvh_root.bound(scene->Shapes);
+   scene->bvh_root.compact();
+   const auto t1 = std::chrono::steady_clock::now();
+   std::fprintf(stderr, "ref_driver: bvh %u nodes, %zu pairs, build %.3f s\n",
+                scene->bvh_root.compact_root->num_nodes,
+                scene->bvh_root.compact_root->leaf_ordered_indices.size(),
+                std::chrono::duration<double>(t1 - t0).count());
+
+   const poly::compact_bvh* cb = scene->bvh_root.compact_root;
+   if (!nodes_out.empty())
+      write_file(nodes_out, cb->nodes, sizeof(poly::compact_bvh_node) * cb->num_nodes);
+   if (!pairs_out.empty()) {
+      std::vector<uint32_t> flat;
+      flat.reserve(cb->leaf_ordered_indices.size() * 2);
+      for (const auto& p : cb->leaf_ordered_indices) { flat.push_back(p.first); flat.push_back(p.second); }
+      write_file(pairs_out, flat.data(), flat.size() * 4);
+   }
+
+   auto load_rays = [](const std::string& path) {
+      Reader r(path.c_str());
+      const uint32_t n = r.u32();
+      std::vector<float> rays(size_t(n) * 6);
+      r.raw(rays.data(), rays.size() * 4);
+      return rays;
+   };
+
+   if (!rays_in.empty() && !hits_out.empty()) {
+      std::vector<float> rays = load_rays(rays_in);
+      const size_t n = rays.size() / 6;
+      std::vector<HitRecord> out(n);
+      for (size_t i = 0; i < n; i++) {
+         poly::Ray ray(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2],
+                       rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+         poly::Intersection is = scene->intersect(ray, -1, -1);
+         HitRecord& h = out[i];
+         std::memset(&h, 0, sizeof(h));
+         h.hit = is.Hits ? 1u : 0u;
+         if (is.Hits) {
+            uint32_t mi = 0;
+            for (; mi < scene->Shapes.size(); mi++) if (scene->Shapes[mi] == is.Shape) break;
+            h.mesh = mi; h.face = is.faceIndex;
+            h.t = ray.MinT; h.u = is.u; h.v = is.v; h.w = is.w;
+            h.loc[0] = is.Location.x; h.loc[1] = is.Location.y; h.loc[2] = is.Location.z;
+            h.geo_n[0] = is.geo_normal.x; h.geo_n[1] = is.geo_normal.y; h.geo_n[2] = is.geo_normal.z;
+            h.t1[0] = is.Tangent1.x; h.t1[1] = is.Tangent1.y; h.t1[2] = is.Tangent1.z;
+            h.t2[0] = is.Tangent2.x; h.t2[1] = is.Tangent2.y; h.t2[2] = is.Tangent2.z;
+            h.err[0] = is.error.x; h.err[1] = is.error.y; h.err[2] = is.error.z;
+         } else {
+            h.mesh = 0xFFFFFFFFu; h.face = 0xFFFFFFFFu;
+         }
+         h.bb_hits = is.num_bb_hits; h.bb_misses = is.num_bb_misses; h.tri_isects = is.num_triangle_isects;
+      }
+      write_file(hits_out, out.data(), out.size() * sizeof(HitRecord));
+   }
+
+   if (!time_rays.empty()) {
+      // CPU baseline: contiguous chunks of the ray list per thread, as a ray-batch
+      // analogue of the reference's per-thread tile loop (src/cpu/main.cpp:187-211).
+      std::vector<float> rays = load_rays(time_rays);
+      const size_t n = rays.size() / 6;
+      std::atomic<uint64_t> checksum{0};
+      const auto s = std::chrono::steady_clock::now();
+      for (unsigned rep = 0; rep < repeat; rep++) {
+         std::vector<std::thread> pool;
+         for (unsigned t = 0; t < threads; t++) {
+            pool.emplace_back([&, t]() {
+               const size_t lo = n * t / threads, hi = n * (t + 1) / threads;
+               uint64_t local = 0;
+               for (size_t i = lo; i < hi; i++) {
+                  poly::Ray ray(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2],
+                                rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+                  poly::Intersection is = scene->intersect(ray, -1, -1);
+                  local += is.Hits ? is.faceIndex : 0xFFFFFFFFu;
+               }
+               checksum += local;
+            });
+         }
+         for (auto& th : pool) th.join();
+      }
+      const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - s).count();
+      std::printf("{\"rays\": %zu, \"repeat\": %u, \"threads\": %u, \"seconds\": %.6f, \"mrays_per_s\": %.4f, \"checksum\": %llu}\n",
+                  n, repeat, threads, secs, double(n) * repeat / secs / 1e6,
+                  (unsigned long long)checksum.load());
+   }
+
+   if (!render_out.empty()) {
+      // All samples of a pixel run in sequence (AbstractRunner::Trace); pixels are
+      // split over threads by rows.  Lambertian scenes use the reference's
+      // unseeded thread-local RNG, so only mirror/emitter/skybox scenes are
+      // reproducible.
+      const unsigned W = runner->Bounds.x, H = runner->Bounds.y;
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < threads; t++) {
+         pool.emplace_back([&, t]() {
+            for (unsigned y = t; y < H; y += threads)
+               for (unsigned x = 0; x < W; x++) runner->Trace((int)x, (int)y);
+         });
+      }
+      for (auto& th : pool) th.join();
+      std::vector<float> film(size_t(W) * H * 3);
+      for (unsigned y = 0; y < H; y++)
+         for (unsigned x = 0; x < W; x++) {
+            poly::SpectralPowerDistribution spd = runner->Film->Filter->Output(poly::Point2i((int)x, (int)y));
+            float* p = &film[(size_t(y) * W + x) * 3];
+            p[0] = spd.r; p[1] = spd.g; p[2] = spd.b;
+         }
+      write_file(render_out, film.data(), film.size() * 4);
+   }
+   return 0;
+}

@@ -1,0 +1,198 @@
+"""polytope_amd — MI355X-native path-tracing hot path for polytope scenes.
+
+Python face of libptx_hip.so (C ABI in include/ptx.h).  The host-side objects
+mirror the reference's scene API: a :class:`Scene` is built from meshes
+(``mesh_geometry`` + ``Mesh``: packed vertices, faces, object-to-world
+transform, Lambert/mirror material, optional emission) and an optional
+constant skybox; its BVH is the reference's binned-SAH tree
+(src/cpu/acceleration/bvh.cpp), built on the host.  Queries and renders run
+on the GPU only: there is no CPU fallback, and a missing library or device
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _abi
+from ._abi import (BVH_NODE_DTYPE, COUNTER_DTYPE, FRAME_DTYPE, HIT_DTYPE, PtxError, check, load)
+from .scenes import Camera, MeshData
+
+__all__ = ["Device", "Scene", "PtxError", "HIT_DTYPE", "COUNTER_DTYPE", "FRAME_DTYPE", "BVH_NODE_DTYPE",
+           "mesh_descs", "build_bvh", "camera_desc", "job_desc", "MeshData", "Camera"]
+
+
+def _ptr(a: Optional[np.ndarray], ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def mesh_descs(meshes: Sequence[MeshData]) -> Tuple[C.Array, list]:
+    """Builds the ptx_mesh_desc array; returns it with the numpy buffers it points into."""
+    keep = []
+    arr = (_abi.MeshDesc * len(meshes))()
+    for i, m in enumerate(meshes):
+        v = np.ascontiguousarray(m.verts, np.float32)
+        f = np.ascontiguousarray(m.faces, np.uint32)
+        xs = [np.ascontiguousarray(v[:, k]) for k in range(3)]
+        fs = [np.ascontiguousarray(f[:, k]) for k in range(3)]
+        ns = [None, None, None]
+        if m.normals is not None:
+            nn = np.ascontiguousarray(m.normals, np.float32)
+            ns = [np.ascontiguousarray(nn[:, k]) for k in range(3)]
+        keep += xs + fs + [n for n in ns if n is not None]
+        d = arr[i]
+        d.num_vertices = v.shape[0]
+        d.num_faces = f.shape[0]
+        d.x, d.y, d.z = (_ptr(x, C.c_float) for x in xs)
+        d.nx, d.ny, d.nz = (_ptr(x, C.c_float) for x in ns)
+        d.fv0, d.fv1, d.fv2 = (_ptr(x, C.c_uint32) for x in fs)
+        d.object_to_world[:] = [float(x) for x in np.asarray(m.object_to_world, np.float32).ravel()]
+        d.world_to_object[:] = [float(x) for x in m.world_to_object().ravel()]
+        d.brdf = int(m.brdf)
+        d.refl[:] = [float(x) for x in m.refl]
+        d.is_light = 1 if m.is_light else 0
+        d.emission[:] = [float(x) for x in m.emission]
+    return arr, keep
+
+
+def camera_desc(cam: Camera) -> _abi.CameraDesc:
+    c = _abi.CameraDesc()
+    c.camera_to_world[:] = [float(x) for x in np.asarray(cam.camera_to_world, np.float32).ravel()]
+    c.fov = cam.fov
+    c.width = cam.width
+    c.height = cam.height
+    c.one_over_width, c.one_over_height, c.aspect_ratio, c.tan_fov_over_2 = cam.derived()
+    return c
+
+
+def job_desc(cam: Camera, spp: int, max_depth: int, seed: int) -> _abi.JobDesc:
+    j = _abi.JobDesc()
+    j.camera = camera_desc(cam)
+    j.spp = spp
+    j.max_depth = max_depth
+    j.seed = seed
+    return j
+
+
+def build_bvh(meshes: Sequence[MeshData], threads: int = 0) -> Tuple[np.ndarray, np.ndarray]:
+    """The reference BVH of ``meshes`` (compact nodes, (P, 2) leaf pairs), built on
+    the host by the library's builder (ptx_bvh_build); needs no GPU."""
+    lib = load()
+    arr, keep = mesh_descs(meshes)
+    nn, np_ = C.c_uint32(0), C.c_uint32(0)
+    marr = C.cast(arr, C.POINTER(_abi.MeshDesc))
+    check(lib.ptx_bvh_build(marr, len(meshes), threads, None, C.byref(nn), None, C.byref(np_)))
+    nodes = np.zeros(nn.value, BVH_NODE_DTYPE)
+    pairs = np.zeros((np_.value, 2), np.uint32)
+    check(lib.ptx_bvh_build(marr, len(meshes), threads, nodes.ctypes.data, C.byref(nn), pairs.ctypes.data,
+                            C.byref(np_)))
+    return nodes, pairs
+
+
+class Device:
+    """An opened gfx950 device (ptx_device_open)."""
+
+    def __init__(self, ordinal: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.ptx_device_open(ordinal, C.byref(h)))
+        self.handle = h
+        self.ordinal = ordinal
+
+    def close(self) -> None:
+        if self.handle:
+            load().ptx_device_close(self.handle)
+            self.handle = None
+
+    def set_tuning(self, paths_per_wave: int = 0, block_size: int = 0, lds_stack: int = 0) -> None:
+        t = _abi.Tuning(paths_per_wave, block_size, lds_stack)
+        check(load().ptx_set_tuning(self.handle, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = _abi.Tuning()
+        check(load().ptx_get_tuning(self.handle, C.byref(t)))
+        return {"paths_per_wave": t.paths_per_wave, "block_size": t.block_size, "lds_stack": t.lds_stack}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A scene resident on the device (ptx_scene_create).
+
+    ``bvh``: optional prebuilt reference BVH as (nodes structured array of
+    BVH_NODE_DTYPE, pairs uint32 array of shape (P, 2)); by default the library
+    builds it on the host."""
+
+    def __init__(self, device: Device, meshes: Sequence[MeshData], skybox: Optional[Sequence[float]] = None,
+                 bvh: Optional[Tuple[np.ndarray, np.ndarray]] = None, build_threads: int = 0):
+        lib = load()
+        self.device = device
+        self.meshes = list(meshes)
+        arr, keep = mesh_descs(self.meshes)
+        desc = _abi.SceneDesc()
+        desc.meshes = C.cast(arr, C.POINTER(_abi.MeshDesc))
+        desc.num_meshes = len(self.meshes)
+        if bvh is not None:
+            nodes = np.ascontiguousarray(bvh[0]).view(BVH_NODE_DTYPE)
+            pairs = np.ascontiguousarray(bvh[1], np.uint32).reshape(-1)
+            keep += [nodes, pairs]
+            desc.nodes = C.cast(nodes.ctypes.data, C.POINTER(_abi.BvhNode))
+            desc.num_nodes = nodes.shape[0]
+            desc.leaf_pairs = pairs.ctypes.data_as(C.POINTER(C.c_uint32))
+            desc.num_pairs = pairs.shape[0] // 2
+        desc.has_skybox = 1 if skybox is not None else 0
+        if skybox is not None:
+            desc.skybox[:] = [float(x) for x in skybox]
+        desc.build_threads = build_threads
+        h = C.c_void_p()
+        check(lib.ptx_scene_create(device.handle, C.byref(desc), C.byref(h)))
+        self.handle = h
+        self.skybox = skybox
+
+    def info(self) -> dict:
+        i = _abi.SceneInfo()
+        check(load().ptx_scene_get_info(self.handle, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in _abi.SceneInfo._fields_}
+
+    def bvh(self) -> Tuple[np.ndarray, np.ndarray]:
+        """The reference-layout BVH (compact nodes, (P, 2) leaf pairs) the scene uses."""
+        inf = self.info()
+        nodes = np.zeros(inf["num_nodes"], BVH_NODE_DTYPE)
+        pairs = np.zeros((inf["num_pairs"], 2), np.uint32)
+        check(load().ptx_scene_copy_bvh(self.handle, nodes.ctypes.data, pairs.ctypes.data))
+        return nodes, pairs
+
+    def intersect(self, rays: np.ndarray, counters: bool = False):
+        """Closest hit of host rays (n, 6) float32 -> structured hits (and counters)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros(n, HIT_DTYPE)
+        cnt = np.zeros(n, COUNTER_DTYPE) if counters else None
+        check(load().ptx_intersect_closest(self.handle, rays.ctypes.data, n, hits.ctypes.data,
+                                           cnt.ctypes.data if cnt is not None else None))
+        return (hits, cnt) if counters else hits
+
+    def intersect_device(self, rays_ptr: int, n: int, hits_ptr: int, counters_ptr: int = 0,
+                         frames_ptr: int = 0, stream: int = 0) -> None:
+        """Closest hit on device buffers (raw device pointers, e.g. torch ``data_ptr()``)."""
+        check(load().ptx_intersect_closest_device(self.handle, rays_ptr or None, n, hits_ptr or None,
+                                                  counters_ptr or None, frames_ptr or None, stream or None))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            load().ptx_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

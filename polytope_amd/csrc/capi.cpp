@@ -1,0 +1,273 @@
+// extern "C" entry points of libptx_hip.so for devices, scenes and ray
+// queries (declared in include/ptx.h); rendering and serving live in
+// render_host.cpp and server.cpp.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "scene.h"
+
+namespace ptx {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string &msg) {
+   g_last_error = msg;
+   return code;
+}
+
+const char *last_error_cstr() { return g_last_error.c_str(); }
+
+double now_s() {
+   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+namespace {
+
+template <class T>
+int upload(ptx_scene *s, const std::vector<T> &v, const T **dst) {
+   *dst = nullptr;
+   if (v.empty()) return PTX_OK;
+   void *p = nullptr;
+   PTX_HIP_TRY(hipMalloc(&p, v.size() * sizeof(T)));
+   s->allocations.push_back(p);
+   PTX_HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+   s->info.device_bytes += v.size() * sizeof(T);
+   *dst = static_cast<const T *>(p);
+   return PTX_OK;
+}
+
+}  // namespace
+
+uint32_t lds_depth_of(const ptx_device *dev) { return dev->tuning.lds_stack ? dev->tuning.lds_stack : 12; }
+
+uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds) {
+   return (uint32_t)(dev->num_cus * closest_hit_blocks_per_cu(lds));
+}
+
+size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds) {
+   const size_t spill_entries = s->d.max_stack > lds ? size_t(s->d.max_stack - lds) * grid * 256 : 0;
+   return 256 + spill_entries * sizeof(uint2);
+}
+
+// Closest-hit launch with stack-spill scratch taken from the stream-ordered allocator.
+int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
+                    ptx_hit_frame *d_frames, hipStream_t stream) {
+   ptx_device *dev = s->dev;
+   const uint32_t lds = lds_depth_of(dev);
+   uint32_t grid = closest_hit_grid(dev, lds);
+   const uint32_t needed_blocks = (n + 255) / 256;
+   grid = std::max(1u, std::min(grid, needed_blocks));
+   void *scratch = nullptr;
+   PTX_HIP_TRY(hipMallocAsync(&scratch, closest_hit_scratch_bytes(s, grid, lds), stream));
+   uint32_t *work = static_cast<uint32_t *>(scratch);
+   uint2 *spill = reinterpret_cast<uint2 *>(static_cast<char *>(scratch) + 256);
+   const uint32_t refill = dev->tuning.block_size ? 16 : 16;
+   const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill,
+                                     stream);
+   const hipError_t fe = hipFreeAsync(scratch, stream);
+   if (rc != 0) return set_error(PTX_ERR_HIP, "closest-hit launch failed");
+   if (fe != hipSuccess) return set_error(PTX_ERR_HIP, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+   return PTX_OK;
+}
+
+}  // namespace ptx
+
+using namespace ptx;
+
+extern "C" {
+
+const char *ptx_last_error(void) { return last_error_cstr(); }
+
+const char *ptx_version(void) { return "ptx_hip 0.1 gfx950"; }
+
+int ptx_device_open(int ordinal, ptx_device **out) {
+   if (!out) return set_error(PTX_ERR_INVALID, "out is NULL");
+   *out = nullptr;
+   int count = 0;
+   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+      return set_error(PTX_ERR_NO_DEVICE, "no HIP device");
+   if (ordinal < 0 || ordinal >= count) return set_error(PTX_ERR_NO_DEVICE, "device ordinal out of range");
+   hipDeviceProp_t prop;
+   PTX_HIP_TRY(hipGetDeviceProperties(&prop, ordinal));
+   std::string arch = prop.gcnArchName;
+   if (arch.rfind("gfx950", 0) != 0)
+      return set_error(PTX_ERR_NO_DEVICE, "device " + std::to_string(ordinal) + " is " + arch +
+                                              ", this library is built for gfx950 only");
+   PTX_HIP_TRY(hipSetDevice(ordinal));
+   auto *d = new ptx_device();
+   d->ordinal = ordinal;
+   d->num_cus = prop.multiProcessorCount;
+   d->arch = arch;
+   if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete d;
+      return set_error(PTX_ERR_HIP, "hipStreamCreate failed");
+   }
+   *out = d;
+   return PTX_OK;
+}
+
+void ptx_device_close(ptx_device *dev) {
+   if (!dev) return;
+   (void)hipSetDevice(dev->ordinal);
+   if (dev->stream) (void)hipStreamDestroy(dev->stream);
+   delete dev;
+}
+
+int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **out) {
+   if (!dev || !desc || !out) return set_error(PTX_ERR_INVALID, "NULL argument");
+   *out = nullptr;
+   if (desc->num_meshes == 0 || !desc->meshes) return set_error(PTX_ERR_INVALID, "scene has no meshes");
+   for (uint32_t i = 0; i < desc->num_meshes; i++) {
+      const ptx_mesh_desc &m = desc->meshes[i];
+      if (!m.x || !m.y || !m.z || !m.fv0 || !m.fv1 || !m.fv2)
+         return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " lacks vertex or face arrays");
+      if (m.brdf != PTX_BRDF_LAMBERT && m.brdf != PTX_BRDF_MIRROR)
+         return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " has an unknown brdf");
+      for (uint32_t f = 0; f < m.num_faces; f++)
+         if (m.fv0[f] >= m.num_vertices || m.fv1[f] >= m.num_vertices || m.fv2[f] >= m.num_vertices)
+            return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " face " + std::to_string(f) +
+                                                  " indexes past its vertices");
+   }
+   PTX_HIP_TRY(hipSetDevice(dev->ordinal));
+   const double t0 = now_s();
+   auto s = std::make_unique<ptx_scene>();
+   s->dev = dev;
+   s->num_meshes = desc->num_meshes;
+   std::vector<WorldMesh> meshes(desc->num_meshes);
+   for (uint32_t i = 0; i < desc->num_meshes; i++) world_vertices(desc->meshes[i], meshes[i]);
+   double build_s = 0;
+   if (desc->nodes && desc->leaf_pairs) {
+      s->bvh.nodes.assign(desc->nodes, desc->nodes + desc->num_nodes);
+      s->bvh.pairs.assign(desc->leaf_pairs, desc->leaf_pairs + 2 * size_t(desc->num_pairs));
+   } else {
+      build_reference_bvh(meshes, desc->build_threads, s->bvh);
+      build_s = now_s() - t0;
+   }
+   HostImage img;
+   const int lrc = layout_scene(meshes, s->bvh, img);
+   if (lrc != PTX_OK) return lrc;
+   const double t1 = now_s();
+   int rc;
+   if ((rc = upload(s.get(), img.nodes, &s->d.nodes)) != PTX_OK) return rc;
+   if ((rc = upload(s.get(), img.leaves, &s->d.leaves)) != PTX_OK) return rc;
+   if ((rc = upload(s.get(), img.tris, &s->d.tris)) != PTX_OK) return rc;
+   if ((rc = upload(s.get(), s->bvh.pairs, &s->d.pairs)) != PTX_OK) return rc;
+   if ((rc = upload(s.get(), img.meshes, &s->d.meshes)) != PTX_OK) return rc;
+   if ((rc = upload(s.get(), img.normals, &s->d.normals)) != PTX_OK) return rc;
+   std::copy(img.root_box, img.root_box + 6, s->d.root_box);
+   s->d.root_ref = img.root_ref;
+   s->d.num_tris = (uint32_t)(img.tris.size() / 9);
+   s->d.max_stack = img.max_stack;
+   s->d.has_sky = desc->has_skybox;
+   std::copy(desc->skybox, desc->skybox + 3, s->d.sky);
+   s->info.num_nodes = (uint32_t)s->bvh.nodes.size();
+   s->info.num_pairs = (uint32_t)(s->bvh.pairs.size() / 2);
+   s->info.num_interior = img.num_interior;
+   s->info.num_leaves = img.num_leaves;
+   s->info.max_depth = img.max_stack;
+   s->info.build_seconds = build_s;
+   s->info.upload_seconds = now_s() - t1;
+   *out = s.release();
+   return PTX_OK;
+}
+
+void ptx_scene_destroy(ptx_scene *scene) {
+   if (!scene) return;
+   (void)hipSetDevice(scene->dev->ordinal);
+   for (void *p : scene->allocations) (void)hipFree(p);
+   delete scene;
+}
+
+int ptx_scene_get_info(const ptx_scene *scene, ptx_scene_info *out) {
+   if (!scene || !out) return set_error(PTX_ERR_INVALID, "NULL argument");
+   *out = scene->info;
+   return PTX_OK;
+}
+
+int ptx_scene_copy_bvh(const ptx_scene *scene, ptx_bvh_node *nodes, uint32_t *leaf_pairs) {
+   if (!scene) return set_error(PTX_ERR_INVALID, "NULL scene");
+   if (nodes) std::memcpy(nodes, scene->bvh.nodes.data(), scene->bvh.nodes.size() * sizeof(ptx_bvh_node));
+   if (leaf_pairs) std::memcpy(leaf_pairs, scene->bvh.pairs.data(), scene->bvh.pairs.size() * sizeof(uint32_t));
+   return PTX_OK;
+}
+
+int ptx_bvh_build(const ptx_mesh_desc *meshes, uint32_t num_meshes, uint32_t threads, ptx_bvh_node *nodes,
+                  uint32_t *num_nodes, uint32_t *leaf_pairs, uint32_t *num_pairs) {
+   if (!meshes || !num_meshes || !num_nodes || !num_pairs) return set_error(PTX_ERR_INVALID, "NULL argument");
+   std::vector<WorldMesh> wm(num_meshes);
+   for (uint32_t i = 0; i < num_meshes; i++) world_vertices(meshes[i], wm[i]);
+   HostBvh bvh;
+   build_reference_bvh(wm, threads, bvh);
+   const uint32_t nn = (uint32_t)bvh.nodes.size(), np = (uint32_t)(bvh.pairs.size() / 2);
+   if (nodes || leaf_pairs) {
+      if (*num_nodes < nn || *num_pairs < np) {
+         *num_nodes = nn;
+         *num_pairs = np;
+         return set_error(PTX_ERR_INVALID, "output buffers too small");
+      }
+      if (nodes) std::memcpy(nodes, bvh.nodes.data(), size_t(nn) * sizeof(ptx_bvh_node));
+      if (leaf_pairs) std::memcpy(leaf_pairs, bvh.pairs.data(), bvh.pairs.size() * sizeof(uint32_t));
+   }
+   *num_nodes = nn;
+   *num_pairs = np;
+   return PTX_OK;
+}
+
+int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t n, ptx_hit *d_hits,
+                                 ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, void *stream) {
+   if (!scene || (n && (!d_rays || !d_hits))) return set_error(PTX_ERR_INVALID, "NULL argument");
+   PTX_HIP_TRY(hipSetDevice(scene->dev->ordinal));
+   return run_closest_hit(scene, d_rays, n, d_hits, d_counters, d_frames, static_cast<hipStream_t>(stream));
+}
+
+int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_hit *hits,
+                          ptx_ray_counters *counters) {
+   if (!scene || (n && (!rays || !hits))) return set_error(PTX_ERR_INVALID, "NULL argument");
+   if (n == 0) return PTX_OK;
+   ptx_device *dev = scene->dev;
+   PTX_HIP_TRY(hipSetDevice(dev->ordinal));
+   std::lock_guard<std::mutex> lock(dev->mu);
+   hipStream_t st = dev->stream;
+   float *d_rays = nullptr;
+   ptx_hit *d_hits = nullptr;
+   ptx_ray_counters *d_cnt = nullptr;
+   PTX_HIP_TRY(hipMallocAsync((void **)&d_rays, size_t(n) * 24, st));
+   PTX_HIP_TRY(hipMallocAsync((void **)&d_hits, size_t(n) * sizeof(ptx_hit), st));
+   if (counters) PTX_HIP_TRY(hipMallocAsync((void **)&d_cnt, size_t(n) * sizeof(ptx_ray_counters), st));
+   PTX_HIP_TRY(hipMemcpyAsync(d_rays, rays, size_t(n) * 24, hipMemcpyHostToDevice, st));
+   int rc = run_closest_hit(scene, d_rays, n, d_hits, d_cnt, nullptr, st);
+   if (rc == PTX_OK) {
+      PTX_HIP_TRY(hipMemcpyAsync(hits, d_hits, size_t(n) * sizeof(ptx_hit), hipMemcpyDeviceToHost, st));
+      if (counters)
+         PTX_HIP_TRY(hipMemcpyAsync(counters, d_cnt, size_t(n) * sizeof(ptx_ray_counters), hipMemcpyDeviceToHost, st));
+   }
+   (void)hipFreeAsync(d_rays, st);
+   (void)hipFreeAsync(d_hits, st);
+   if (d_cnt) (void)hipFreeAsync(d_cnt, st);
+   PTX_HIP_TRY(hipStreamSynchronize(st));
+   return rc;
+}
+
+int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
+   if (!dev || !t) return set_error(PTX_ERR_INVALID, "NULL argument");
+   if (t->lds_stack && t->lds_stack != 8 && t->lds_stack != 12 && t->lds_stack != 16 && t->lds_stack != 24)
+      return set_error(PTX_ERR_INVALID, "lds_stack must be 0, 8, 12, 16 or 24");
+   dev->tuning = *t;
+   return PTX_OK;
+}
+
+int ptx_get_tuning(ptx_device *dev, ptx_tuning *t) {
+   if (!dev || !t) return set_error(PTX_ERR_INVALID, "NULL argument");
+   *t = dev->tuning;
+   if (!t->lds_stack) t->lds_stack = lds_depth_of(dev);
+   if (!t->block_size) t->block_size = 256;
+   return PTX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// Internal declarations shared by the host C++ and HIP translation units of
+// libptx_hip.so.  Not part of the C ABI (include/ptx.h).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ptx.h"
+
+namespace ptx {
+
+// ---------------------------------------------------------------- device layout
+//
+// Interior node record, 64 bytes, one per interior node of the reference BVH in
+// DFS order.  Child 0 is the reference's low child (node i+1), child 1 its high
+// child (node i + high_child_offset).  A child reference with bit 31 set names
+// a leaf (index into the leaf table), otherwise an interior record.
+//   f[0..5]   child 0 box p0.xyz, p1.xyz
+//   f[6..11]  child 1 box p0.xyz, p1.xyz
+//   u[12]     child 0 ref, u[13] child 1 ref, u[14] split axis, u[15] unused
+struct alignas(64) Node64 {
+   float box[12];
+   uint32_t ref[2];
+   uint32_t axis;
+   uint32_t pad;
+};
+static_assert(sizeof(Node64) == 64, "Node64 is 64 bytes");
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// Leaf table entry: first triangle (index into the leaf-ordered triangle
+// array, which is the reference's leaf_ordered_indices order) and the count
+// (the reference's u16 num_faces).
+struct LeafRec {
+   uint32_t first, count;
+};
+
+// Per-mesh shading record.
+struct MeshRec {
+   int32_t brdf;
+   float refl[3];
+   int32_t is_light;
+   float emission[3];
+   uint32_t has_normals;
+   uint32_t normal_base;   // index of the mesh's first corner normal / 3 (per face)
+};
+
+struct DevScene {
+   const Node64 *nodes;
+   const LeafRec *leaves;
+   const float *tris;        // 9 floats (world v0, v1, v2) per leaf-ordered triangle
+   const uint32_t *pairs;    // (mesh, face) per leaf-ordered triangle
+   const MeshRec *meshes;
+   const float *normals;     // world corner normals, 9 floats per leaf-ordered triangle (or null)
+   float root_box[6];
+   uint32_t root_ref;        // kLeafBit|leaf or interior index
+   uint32_t num_tris;
+   uint32_t max_stack;       // deepest deferred-stack use + 1 (tree depth bound)
+   int32_t has_sky;
+   float sky[3];
+};
+
+struct CameraParams {
+   float m[16];
+   float oow, ooh, aspect, tanf2;
+   uint32_t width, height;
+};
+
+// ---------------------------------------------------------------- host-side scene
+
+struct HostBvh {
+   std::vector<ptx_bvh_node> nodes;       // reference compact layout
+   std::vector<uint32_t> pairs;           // 2 per leaf entry (mesh, face)
+};
+
+struct WorldMesh {
+   uint32_t nf = 0;
+   std::vector<float> wv;                 // 9 floats per face (world v0, v1, v2)
+   bool has_normals = false;
+   std::vector<float> wn;                 // 9 floats per face (world corner normals)
+   MeshRec rec{};
+};
+
+// Transform::ApplyInPlace(Point&) semantics (src/cpu/structures/Transform.cpp:69-91).
+void world_vertices(const ptx_mesh_desc &m, WorldMesh &out);
+
+// Decision-identical re-implementation of bvh::bound + bvh::compact
+// (src/cpu/acceleration/bvh.cpp:36-395, 652-698), multithreaded.
+void build_reference_bvh(const std::vector<WorldMesh> &meshes, unsigned threads, HostBvh &out);
+
+// Error reporting: per-thread message returned by ptx_last_error().
+int set_error(int code, const std::string &msg);
+
+}  // namespace ptx

@@ -1,0 +1,64 @@
+// Host-side scene image and the device scene object behind ptx_scene.
+#pragma once
+
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "launch.h"
+#include "ptx_internal.h"
+
+#define PTX_HIP_TRY(expr)                                                                           \
+   do {                                                                                             \
+      hipError_t e_ = (expr);                                                                       \
+      if (e_ != hipSuccess)                                                                         \
+         return ::ptx::set_error(PTX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+   } while (0)
+
+struct ptx_device;
+struct ptx_scene;
+
+namespace ptx {
+
+struct HostImage {
+   std::vector<Node64> nodes;
+   std::vector<LeafRec> leaves;
+   std::vector<float> tris;
+   std::vector<float> normals;
+   std::vector<MeshRec> meshes;
+   float root_box[6] = {0, 0, 0, 0, 0, 0};
+   uint32_t root_ref = 0;
+   uint32_t max_stack = 1;
+   uint32_t num_interior = 0, num_leaves = 0;
+};
+
+int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostImage &img);
+
+double now_s();
+const char *last_error_cstr();
+uint32_t lds_depth_of(const ptx_device *dev);
+uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds);
+size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds);
+int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
+                    ptx_hit_frame *d_frames, hipStream_t stream);
+
+}  // namespace ptx
+
+struct ptx_device {
+   int ordinal = 0;
+   int num_cus = 0;
+   std::string arch;
+   hipStream_t stream = nullptr;     // library stream (default priority)
+   ptx_tuning tuning{};
+   std::mutex mu;
+};
+
+struct ptx_scene {
+   ptx_device *dev = nullptr;
+   ptx::HostBvh bvh;                 // reference-layout BVH (kept for ptx_scene_copy_bvh)
+   ptx::DevScene d{};                // device pointers + root
+   std::vector<void *> allocations;
+   ptx_scene_info info{};
+   uint32_t num_meshes = 0;
+};

@@ -1,0 +1,416 @@
+"""Seeded synthetic scenes and ray sets for the closest-hit and render workloads.
+
+Every generator is deterministic from its seed and uses only IEEE basic
+operations (add, mul, div, sqrt) and integer hashing, so the same call yields
+the same bits on any host: sines and cosines come from ``_sincos64`` below, a
+fixed-term Taylor evaluation in float64, not from libm.
+
+Scenes are lists of :class:`MeshData`, the host-side analogue of the
+reference's ``mesh_geometry`` + ``Mesh`` pair (src/cpu/shapes/mesh.h:17-113):
+packed vertices, three face-index arrays, a 4x4 object-to-world matrix and a
+material (Lambert ``refl`` or perfect mirror), optionally an emitter.
+"""
+from __future__ import annotations
+
+import math
+import struct
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+LAMBERT = 1
+MIRROR = 2
+
+IDENTITY = np.eye(4, dtype=np.float32)
+
+
+@dataclass
+class MeshData:
+    verts: np.ndarray                      # (V, 3) float32, packed vertices
+    faces: np.ndarray                      # (F, 3) uint32
+    object_to_world: np.ndarray = field(default_factory=lambda: IDENTITY.copy())
+    normals: Optional[np.ndarray] = None   # (V, 3) float32 vertex normals
+    brdf: int = LAMBERT
+    refl: Tuple[float, float, float] = (0.5, 0.5, 0.5)
+    is_light: bool = False
+    emission: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+
+    @property
+    def num_faces(self) -> int:
+        return int(self.faces.shape[0])
+
+    def world_to_object(self) -> np.ndarray:
+        return np.linalg.inv(self.object_to_world.astype(np.float64)).astype(np.float32)
+
+
+@dataclass
+class Camera:
+    """Pinhole camera: camera-to-world matrix plus the values
+    ``PerspectiveCamera``'s constructor derives (src/cpu/cameras/PerspectiveCamera.cpp:17-25)."""
+    camera_to_world: np.ndarray
+    fov: float
+    width: int
+    height: int
+
+    def derived(self) -> Tuple[float, float, float, float]:
+        f32 = np.float32
+        one_over_w = f32(1.0) / f32(self.width)
+        one_over_h = f32(1.0) / f32(self.height)
+        aspect = f32(self.width) * one_over_h
+        pi_over_180 = f32(math.pi / 180.0)
+        half = f32(self.fov) * pi_over_180 * f32(0.5)
+        tan_f2 = f32(math.tan(float(half)))
+        return float(one_over_w), float(one_over_h), float(aspect), float(tan_f2)
+
+
+# --------------------------------------------------------------------------- maths helpers
+
+def _sincos64(x: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """sin and cos of float64 angles with basic operations only: reduce by
+    k*pi/2 (k = round(x*2/pi)), then Taylor series to degree 21 on [-pi/4, pi/4]
+    (truncation error below 1e-17)."""
+    x = np.asarray(x, dtype=np.float64)
+    k = np.floor(x * (2.0 / math.pi) + 0.5)
+    r = x - k * (math.pi / 2.0)
+    r2 = r * r
+    s = np.zeros_like(r)
+    c = np.zeros_like(r)
+    for n in range(21, 0, -2):          # sin: r - r^3/3! + ...
+        s = s * r2 + ((-1.0) ** ((n - 1) // 2)) / math.factorial(n)
+    s = s * r
+    for n in range(20, -1, -2):         # cos: 1 - r^2/2! + ...
+        c = c * r2 + ((-1.0) ** (n // 2)) / math.factorial(n)
+    q = (k.astype(np.int64)) & 3
+    sin = np.where(q == 0, s, np.where(q == 1, c, np.where(q == 2, -s, -c)))
+    cos = np.where(q == 0, c, np.where(q == 1, -s, np.where(q == 2, -c, s)))
+    return sin, cos
+
+
+def _hash_u32(*keys: np.ndarray) -> np.ndarray:
+    """Integer hash (xorshift-multiply finaliser, as in MurmurHash3 fmix32) of
+    several uint32 keys combined by a multiply-xor chain."""
+    h = np.full(np.broadcast(*keys).shape, 0x9E3779B9, dtype=np.uint64)
+    for k in keys:
+        h = (h ^ np.asarray(k, dtype=np.uint64)) * np.uint64(0x85EBCA6B) & np.uint64(0xFFFFFFFF)
+        h ^= h >> np.uint64(13)
+    h = (h * np.uint64(0xC2B2AE35)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    return h.astype(np.uint32)
+
+
+class Rng:
+    """Counter-based uniform stream: value i of stream s is hash(seed, s, i)."""
+
+    def __init__(self, seed: int):
+        self.seed = np.uint32(seed)
+        self.stream = 0
+
+    def uniform(self, n: int) -> np.ndarray:
+        self.stream += 1
+        i = np.arange(n, dtype=np.uint64)
+        lo = (i & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        hi = (i >> np.uint64(32)).astype(np.uint32)
+        h = _hash_u32(np.uint32(self.seed), np.uint32(self.stream), lo, hi)
+        return (h >> np.uint32(8)).astype(np.float64) * (1.0 / (1 << 24))
+
+
+def value_noise(p: np.ndarray, seed: int, freq: float) -> np.ndarray:
+    """Trilinear value noise in [-1, 1] on an integer lattice scaled by ``freq``,
+    with smoothstep weights; lattice values from an integer hash."""
+    q = p * freq
+    i0 = np.floor(q)
+    f = q - i0
+    w = f * f * (3.0 - 2.0 * f)
+    ii = i0.astype(np.int64)
+    out = np.zeros(p.shape[0], dtype=np.float64)
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                h = _hash_u32(np.uint32(seed),
+                              (ii[:, 0] + dx).astype(np.uint32),
+                              (ii[:, 1] + dy).astype(np.uint32),
+                              (ii[:, 2] + dz).astype(np.uint32))
+                val = (h >> np.uint32(8)).astype(np.float64) * (2.0 / (1 << 24)) - 1.0
+                wx = w[:, 0] if dx else 1.0 - w[:, 0]
+                wy = w[:, 1] if dy else 1.0 - w[:, 1]
+                wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                out += val * wx * wy * wz
+    return out
+
+
+# --------------------------------------------------------------------------- meshes
+
+def icosphere(subdiv: int, radius: float = 1.0, center=(0.0, 0.0, 0.0), noise: float = 0.05,
+              seed: int = 2, noise_freq: float = 3.0) -> Tuple[np.ndarray, np.ndarray]:
+    """Icosahedron subdivided ``subdiv`` times (20 * 4**subdiv faces), vertices
+    projected to the unit sphere, then displaced radially by value noise of
+    amplitude ``noise`` (fraction of the radius)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    v /= np.sqrt((v * v).sum(1))[:, None]
+    for _ in range(subdiv):
+        nv = v.shape[0]
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        edges = np.stack([np.stack([a, b], 1), np.stack([b, c], 1), np.stack([c, a], 1)], 1).reshape(-1, 2)
+        lo = edges.min(1)
+        hi = edges.max(1)
+        key = lo * nv + hi
+        uniq, inv = np.unique(key, return_inverse=True)
+        mid = (v[uniq // nv] + v[uniq % nv]) * 0.5
+        mid /= np.sqrt((mid * mid).sum(1))[:, None]
+        v = np.concatenate([v, mid], 0)
+        m = (inv + nv).reshape(-1, 3)
+        mab, mbc, mca = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([np.stack([a, mab, mca], 1), np.stack([b, mbc, mab], 1),
+                            np.stack([c, mca, mbc], 1), np.stack([mab, mbc, mca], 1)], 0)
+    if noise:
+        v = v * (1.0 + noise * value_noise(v, seed, noise_freq))[:, None]
+    v = v * radius + np.asarray(center, dtype=np.float64)
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def latlong_sphere(n_lat: int = 20, n_lon: int = 40, radius: float = 1.0,
+                   center=(0.0, 0.0, 0.0)) -> Tuple[np.ndarray, np.ndarray]:
+    """Latitude-longitude sphere: (n_lat+1) rings of n_lon vertices (the pole rings
+    collapse to a point), 2*n_lat*n_lon faces including zero-area pole faces."""
+    th = np.arange(n_lat + 1, dtype=np.float64) * (math.pi / n_lat)
+    ph = np.arange(n_lon, dtype=np.float64) * (2.0 * math.pi / n_lon)
+    st, ct = _sincos64(th)
+    sp, cp = _sincos64(ph)
+    x = st[:, None] * cp[None, :]
+    y = np.repeat(ct[:, None], n_lon, 1)
+    z = st[:, None] * sp[None, :]
+    v = np.stack([x, y, z], -1).reshape(-1, 3) * radius + np.asarray(center, dtype=np.float64)
+    i = np.arange(n_lat)[:, None]
+    j = np.arange(n_lon)[None, :]
+    a = i * n_lon + j
+    b = i * n_lon + (j + 1) % n_lon
+    c = (i + 1) * n_lon + j
+    d = (i + 1) * n_lon + (j + 1) % n_lon
+    f = np.concatenate([np.stack([a, c, b], -1).reshape(-1, 3), np.stack([b, c, d], -1).reshape(-1, 3)], 0)
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def quad(half: float, y: float = 0.0) -> Tuple[np.ndarray, np.ndarray]:
+    """Axis-aligned square in the plane y = const, two triangles."""
+    v = np.array([[-half, y, -half], [half, y, -half], [half, y, half], [-half, y, half]], dtype=np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32)
+    return v, f
+
+
+def sliver_grid(n: int, size: float, aspect: float = 1000.0, y: float = 0.0,
+                seed: int = 5) -> Tuple[np.ndarray, np.ndarray]:
+    """Regular (n+1) x (n+1) grid mesh in the plane y=const whose column widths
+    vary so cells reach ``aspect``:1; 2*n*n long thin triangles sharing edges."""
+    rng = Rng(seed)
+    w = 1.0 + (aspect - 1.0) * rng.uniform(n) ** 4
+    xs = np.concatenate([[0.0], np.cumsum(1.0 / w)])
+    xs = xs / xs[-1] * size - size / 2
+    zs = np.linspace(-size / 2, size / 2, n + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")
+    v = np.stack([X.ravel(), np.full(X.size, y), Z.ravel()], 1)
+    i = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    a = i * (n + 1) + j
+    b = a + 1
+    c = a + (n + 1)
+    d = c + 1
+    f = np.concatenate([np.stack([a, c, b], -1).reshape(-1, 3), np.stack([b, c, d], -1).reshape(-1, 3)], 0)
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def translate(t) -> np.ndarray:
+    m = np.eye(4, dtype=np.float32)
+    m[:3, 3] = np.asarray(t, dtype=np.float32)
+    return m
+
+
+def scale(s) -> np.ndarray:
+    s = np.broadcast_to(np.asarray(s, dtype=np.float32), (3,))
+    return np.diag([s[0], s[1], s[2], 1.0]).astype(np.float32)
+
+
+def rigid(seed: int, extent: float) -> np.ndarray:
+    """Seeded rotation (unit quaternion from hashed uniforms) plus translation
+    uniform in [-extent, extent]^3."""
+    u = Rng(seed).uniform(7)
+    s1, c1 = _sincos64(np.array([2 * math.pi * u[1]]))
+    s2, c2 = _sincos64(np.array([2 * math.pi * u[2]]))
+    a, b = math.sqrt(1 - u[0]), math.sqrt(u[0])
+    qx, qy, qz, qw = a * s1[0], a * c1[0], b * s2[0], b * c2[0]
+    r = np.array([[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+                  [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+                  [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]])
+    m = np.eye(4)
+    m[:3, :3] = r
+    m[:3, 3] = (u[3:6] * 2 - 1) * extent
+    return m.astype(np.float32)
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """Camera-to-world matrix for a left-handed camera looking down +z."""
+    eye = np.asarray(eye, dtype=np.float64)
+    d = np.asarray(target, dtype=np.float64) - eye
+    d /= np.linalg.norm(d)
+    right = np.cross(np.asarray(up, dtype=np.float64), d)
+    right /= np.linalg.norm(right)
+    new_up = np.cross(d, right)
+    m = np.eye(4)
+    m[:3, 0] = right
+    m[:3, 1] = new_up
+    m[:3, 2] = d
+    m[:3, 3] = eye
+    return m.astype(np.float32)
+
+
+# --------------------------------------------------------------------------- rays
+
+def rays_coherent(cam: Camera, nx: int, ny: int) -> np.ndarray:
+    """Primary pinhole rays through an nx x ny grid of pixel centres of ``cam``
+    (camera rays as the reference camera computes them, in float32)."""
+    f32 = np.float32
+    oow, ooh, aspect, tanf2 = (f32(x) for x in Camera(cam.camera_to_world, cam.fov, nx, ny).derived())
+    xs = (np.arange(nx, dtype=np.float32) + f32(0.5)) * oow
+    ys = (np.arange(ny, dtype=np.float32) + f32(0.5)) * ooh
+    cx = (f32(2) * xs - f32(1)) * aspect * tanf2
+    cy = (f32(1) - f32(2) * ys) * tanf2
+    CX, CY = np.meshgrid(cx, cy, indexing="xy")
+    d = np.stack([CX.ravel(), CY.ravel(), np.ones(CX.size, np.float32)], 1).astype(np.float32)
+    d = d / np.sqrt((d * d).sum(1, dtype=np.float32))[:, None]
+    m = cam.camera_to_world.astype(np.float32)
+    w = (d @ m[:3, :3].T).astype(np.float32)
+    w = w / np.sqrt((w * w).sum(1, dtype=np.float32))[:, None]
+    o = np.broadcast_to(m[:3, 3], w.shape)
+    return np.concatenate([o, w], 1).astype(np.float32)
+
+
+def unit_vectors(n: int, rng: Rng) -> np.ndarray:
+    z = rng.uniform(n) * 2.0 - 1.0
+    phi = rng.uniform(n) * (2.0 * math.pi)
+    s, c = _sincos64(phi)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    return np.stack([r * c, r * s, z], 1)
+
+
+def rays_incoherent(bmin, bmax, n: int, seed: int) -> np.ndarray:
+    """Origins uniform in the box [bmin, bmax], directions uniform on the sphere."""
+    rng = Rng(seed)
+    bmin = np.asarray(bmin, np.float64)
+    bmax = np.asarray(bmax, np.float64)
+    o = bmin + (bmax - bmin) * np.stack([rng.uniform(n), rng.uniform(n), rng.uniform(n)], 1)
+    d = unit_vectors(n, rng)
+    return np.concatenate([o, d], 1).astype(np.float32)
+
+
+def rays_at_features(verts: np.ndarray, faces: np.ndarray, n: int, seed: int,
+                     dist: float = 3.0, center=None) -> np.ndarray:
+    """Rays aimed exactly at shared mesh features: half at vertices, half at edge
+    midpoints (computed in float32), from origins on a sphere of radius ``dist``
+    around ``center``.  Direction = normalize(target - origin)."""
+    rng = Rng(seed)
+    center = verts.mean(0) if center is None else np.asarray(center)
+    fi = (rng.uniform(n) * faces.shape[0]).astype(np.int64) % faces.shape[0]
+    ci = (rng.uniform(n) * 3).astype(np.int64) % 3
+    a = verts[faces[fi, ci]].astype(np.float32)
+    b = verts[faces[fi, (ci + 1) % 3]].astype(np.float32)
+    mid = ((a + b) * np.float32(0.5)).astype(np.float32)
+    target = np.where((np.arange(n) % 2 == 0)[:, None], a, mid)
+    o = (center + unit_vectors(n, rng) * dist).astype(np.float32)
+    d = (target.astype(np.float64) - o.astype(np.float64))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return np.concatenate([o, d.astype(np.float32)], 1).astype(np.float32)
+
+
+# --------------------------------------------------------------------------- files
+
+def write_ptxm(path: str, meshes: Sequence[MeshData], camera: Optional[Camera] = None,
+               spp: int = 1, max_depth: int = 5, skybox: Optional[Sequence[float]] = None) -> None:
+    """Scene container read by the reference test driver (oracle/ref/ref_driver.cpp)."""
+    out = bytearray()
+    out += struct.pack("<III", 0x4D585450, 1, len(meshes))
+    for m in meshes:
+        v = np.ascontiguousarray(m.verts, np.float32)
+        f = np.ascontiguousarray(m.faces, np.uint32)
+        out += struct.pack("<III", v.shape[0], f.shape[0], 1 if m.normals is not None else 0)
+        out += np.ascontiguousarray(m.object_to_world, np.float32).tobytes()
+        out += struct.pack("<I3fI3f", m.brdf, *m.refl, 1 if m.is_light else 0, *m.emission)
+        for k in range(3):
+            out += np.ascontiguousarray(v[:, k]).tobytes()
+        if m.normals is not None:
+            nn = np.ascontiguousarray(m.normals, np.float32)
+            for k in range(3):
+                out += np.ascontiguousarray(nn[:, k]).tobytes()
+        for k in range(3):
+            out += np.ascontiguousarray(f[:, k]).tobytes()
+    if camera is not None:
+        out += struct.pack("<I", 1) + np.ascontiguousarray(camera.camera_to_world, np.float32).tobytes()
+        out += struct.pack("<fIIII", camera.fov, camera.width, camera.height, spp, max_depth)
+    else:
+        out += struct.pack("<I", 0) + np.eye(4, dtype=np.float32).tobytes() + struct.pack("<fIIII", 90.0, 1, 1, 1, 1)
+    if skybox is not None:
+        out += struct.pack("<I3f", 1, *skybox)
+    else:
+        out += struct.pack("<I3f", 0, 0.0, 0.0, 0.0)
+    with open(path, "wb") as fh:
+        fh.write(bytes(out))
+
+
+def write_rays(path: str, rays: np.ndarray) -> None:
+    rays = np.ascontiguousarray(rays, np.float32)
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<I", rays.shape[0]))
+        fh.write(rays.tobytes())
+
+
+# --------------------------------------------------------------------------- workloads
+
+def scene_bounds(meshes: Sequence[MeshData]) -> Tuple[np.ndarray, np.ndarray]:
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    for m in meshes:
+        w = (np.c_[m.verts.astype(np.float64), np.ones(len(m.verts))] @ m.object_to_world.astype(np.float64).T)[:, :3]
+        lo = np.minimum(lo, w.min(0))
+        hi = np.maximum(hi, w.max(0))
+    return lo, hi
+
+
+def closest_hit_workload(subdiv: int, n_rays: int, seed: int = 2):
+    """Closest-hit microbench: displaced icosphere (radius 1 at the origin) and
+    n_rays rays, half coherent pinhole rays on a 2:1 grid aimed at the sphere,
+    half incoherent (origin uniform in the bounding box, direction uniform)."""
+    v, f = icosphere(subdiv, noise=0.05, seed=seed)
+    meshes = [MeshData(v, f)]
+    n_coh = n_rays // 2
+    nx = int(round(math.sqrt(n_coh * 2)))
+    ny = max(1, n_coh // nx)
+    cam = Camera(look_at((0.0, 0.4, -3.2), (0.0, 0.0, 0.0)), 60.0, nx, ny)
+    coh = rays_coherent(cam, nx, ny)
+    lo, hi = scene_bounds(meshes)
+    inc = rays_incoherent(lo, hi, n_rays - coh.shape[0], seed + 100)
+    return meshes, np.concatenate([coh, inc], 0), coh.shape[0]
+
+
+def plain_render_scene(seed: int = 1, width: int = 640, height: int = 480):
+    """Ground quad plus three 20x40 lat-long spheres (1,600 faces each), radius
+    U[0.5,1.5], Lambert albedo U[0.2,0.8]^3, constant skybox, 60-degree camera."""
+    rng = Rng(seed)
+    u = rng.uniform(32)
+    meshes = []
+    gv, gf = quad(12.0, 0.0)
+    meshes.append(MeshData(gv, gf, refl=(0.6, 0.6, 0.6)))
+    xs = (-2.6, 0.0, 2.6)
+    for i in range(3):
+        r = 0.5 + u[i]
+        sv, sf = latlong_sphere(20, 40, 1.0)
+        o2w = translate((xs[i], r, 1.0 + 2.0 * u[3 + i])) @ scale(r)
+        refl = tuple(float(0.2 + 0.6 * u[6 + 3 * i + k]) for k in range(3))
+        meshes.append(MeshData(sv, sf, object_to_world=o2w.astype(np.float32), refl=refl))
+    cam = Camera(look_at((0.0, 2.2, -6.0), (0.0, 0.8, 1.5)), 60.0, width, height)
+    return meshes, cam, (180.0, 200.0, 230.0)

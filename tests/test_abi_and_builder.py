@@ -1,0 +1,91 @@
+"""CPU checks of the product library: it loads, exports every entry point
+include/ptx.h declares, and its host BVH builder (ptx_bvh_build) takes the
+decisions of the reference builder — checked against the oracle's restatement
+and, at small sizes, the reference's own golden BVHs."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from polytope_amd import _abi, build_bvh
+from polytope_amd import scenes as S
+from tests import _fixtures as F
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "ptx.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ptx_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load()
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in include/ptx.h but not exported"
+        assert n in _abi.SIGNATURES, f"{n} has no ctypes signature"
+    assert lib.ptx_version().decode().startswith("ptx_hip")
+
+
+def test_struct_layouts_match_header():
+    assert C.sizeof(_abi.MeshDesc) == 240
+    assert C.sizeof(_abi.BvhNode) == 32
+    assert C.sizeof(_abi.CameraDesc) == 16 * 4 + 4 + 8 + 16
+    assert _abi.HIT_DTYPE.itemsize == 24
+    assert _abi.FRAME_DTYPE.itemsize == 60
+
+
+def test_device_open_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    lib = _abi.load()
+    h = C.c_void_p()
+    rc = lib.ptx_device_open(0, C.byref(h))
+    assert rc == _abi.PTX_ERR_NO_DEVICE
+    assert lib.ptx_last_error()
+
+
+@pytest.mark.parametrize("name", F.NAMES)
+def test_builder_matches_reference_golden(name):
+    d = F.load(name)
+    nodes, pairs = build_bvh(F.meshes_of(d))
+    assert nodes.tobytes() == d["ref_nodes"].tobytes()
+    assert np.array_equal(pairs, d["ref_pairs"])
+
+
+def _cases():
+    v7, f7 = S.icosphere(7, noise=0.05, seed=2)
+    yield "ico7", [S.MeshData(v7, f7)]
+    meshes = []
+    v4, f4 = S.icosphere(4, noise=0.05, seed=3)
+    for i in range(6):
+        meshes.append(S.MeshData(v4, f4, object_to_world=S.rigid(100 + i, 3.0)))
+    gv, gf = S.quad(8.0, -4.0)
+    meshes.append(S.MeshData(gv, gf))
+    yield "instances", meshes
+    sv, sf = S.sliver_grid(200, 6.0, aspect=1000.0, seed=5)
+    yield "sliver200", [S.MeshData(sv, sf)]
+    # degenerate: many faces with one centroid (stop case 3) and zero-area faces
+    z = np.zeros((300, 3), np.float32)
+    yield "degenerate", [S.MeshData(np.concatenate([z, np.eye(3, dtype=np.float32)]),
+                                    np.concatenate([np.arange(300).reshape(100, 3),
+                                                    np.array([[300, 301, 302]])]).astype(np.uint32))]
+
+
+@pytest.mark.parametrize("case", list(_cases()), ids=lambda c: c[0])
+@pytest.mark.parametrize("threads", [1, 8])
+def test_builder_matches_oracle(case, threads):
+    _, meshes = case
+    nodes, pairs = build_bvh(meshes, threads=threads)
+    on, op = O.OracleScene(meshes).bvh()
+    assert nodes.tobytes() == on.tobytes()
+    assert np.array_equal(pairs, op)
