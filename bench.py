@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--cpu-sample-log2", type=int, default=17)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--lds", type=int, default=0, help="LDS stack entries per lane (0 = library default)")
+    ap.add_argument("--no-coherent", action="store_true", help="skip the coherent-rate measurement")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
 
 
@@ -63,7 +67,7 @@ def cpu_baseline(meshes, rays: np.ndarray, threads: int):
             S.write_rays(f"{d}/r.bin", rays)
             p = subprocess.run([str(O.REF_DRIVER), "--ptxm", f"{d}/s.bin", "--time-rays", f"{d}/r.bin",
                                 "--threads", str(threads)], capture_output=True, text=True, timeout=900, cwd=d)
-            line = [ln for ln in p.stdout.splitlines() if ln.startswith("{\"rays\"")]
+            line = [ln[ln.index("{\"rays\""):] for ln in p.stdout.splitlines() if "{\"rays\"" in ln]
             if p.returncode == 0 and line:
                 r = json.loads(line[-1])
                 return {"value": round(r["mrays_per_s"], 4), "unit": "Mrays/s", "cores": threads,
@@ -95,6 +99,8 @@ def main():
     print(f"[bench] workload built: {meshes[0].num_faces} triangles, {n_rays} rays "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     dev = Device(0)
+    if args.lds:
+        dev.set_tuning(lds_stack=args.lds)
     sc = Scene(dev, meshes)
     info = sc.info()
     print(f"[bench] scene: {info['num_nodes']} nodes, build {info['build_seconds']:.2f} s, "
@@ -113,15 +119,16 @@ def main():
     c = cnt[: n_inc * 3].cpu().numpy().view(COUNTER_DTYPE)
     b_ray = algorithmic_bytes(c)
     mean_b = float(b_ray.mean())
-    # coherent rate (untimed by the step loop)
-    for _ in range(2):
-        sc.intersect_device(d_rays.data_ptr(), n_coh, hits.data_ptr(), 0, 0, stream.cuda_stream)
-    torch.cuda.synchronize()
-    tc = time.perf_counter()
-    for _ in range(5):
-        sc.intersect_device(d_rays.data_ptr(), n_coh, hits.data_ptr(), 0, 0, stream.cuda_stream)
-    torch.cuda.synchronize()
-    coherent_mrays = 5 * n_coh / (time.perf_counter() - tc) / 1e6
+    coherent_mrays = None
+    if not args.no_coherent:                      # coherent rate (outside the step loop)
+        for _ in range(2):
+            sc.intersect_device(d_rays.data_ptr(), n_coh, hits.data_ptr(), 0, 0, stream.cuda_stream)
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        for _ in range(5):
+            sc.intersect_device(d_rays.data_ptr(), n_coh, hits.data_ptr(), 0, 0, stream.cuda_stream)
+        torch.cuda.synchronize()
+        coherent_mrays = round(5 * n_coh / (time.perf_counter() - tc) / 1e6, 2)
 
     for _ in range(args.warmup):
         sc.intersect_device(inc_ptr, n_inc, hits.data_ptr(), 0, 0, stream.cuda_stream)
@@ -174,10 +181,17 @@ def main():
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(mean_b, 1), "kernel": "k_closest_hit",
                      "launch_ms": round(launch_ms, 4)},
-        "coherent_mrays_per_s": round(coherent_mrays, 2),
+        "coherent_mrays_per_s": coherent_mrays,
+        "tuning": dev.get_tuning(),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.calibrate_copy_mb:
+        src = torch.empty(args.calibrate_copy_mb << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {args.calibrate_copy_mb} MiB done", file=sys.stderr, flush=True)
     sc.close()
     dev.close()
 

@@ -52,7 +52,7 @@ uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds) {
 
 size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds) {
    const size_t spill_entries = s->d.max_stack > lds ? size_t(s->d.max_stack - lds) * grid * 256 : 0;
-   return 256 + spill_entries * sizeof(uint2);
+   return 256 + spill_entries * sizeof(uint4);
 }
 
 // Closest-hit launch with stack-spill scratch taken from the stream-ordered allocator.
@@ -66,7 +66,7 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
    void *scratch = nullptr;
    PTX_HIP_TRY(hipMallocAsync(&scratch, closest_hit_scratch_bytes(s, grid, lds), stream));
    uint32_t *work = static_cast<uint32_t *>(scratch);
-   uint2 *spill = reinterpret_cast<uint2 *>(static_cast<char *>(scratch) + 256);
+   uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(scratch) + 256);
    const uint32_t refill = dev->tuning.block_size ? 16 : 16;
    const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill,
                                      stream);
@@ -155,13 +155,13 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    const double t1 = now_s();
    int rc;
    if ((rc = upload(s.get(), img.nodes, &s->d.nodes)) != PTX_OK) return rc;
-   if ((rc = upload(s.get(), img.leaves, &s->d.leaves)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.tris, &s->d.tris)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), s->bvh.pairs, &s->d.pairs)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.meshes, &s->d.meshes)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.normals, &s->d.normals)) != PTX_OK) return rc;
    std::copy(img.root_box, img.root_box + 6, s->d.root_box);
    s->d.root_ref = img.root_ref;
+   s->d.root_count = img.root_count;
    s->d.num_tris = (uint32_t)(img.tris.size() / 9);
    s->d.max_stack = img.max_stack;
    s->d.has_sky = desc->has_skybox;

@@ -15,27 +15,25 @@ namespace ptx {
 //
 // Interior node record, 64 bytes, one per interior node of the reference BVH in
 // DFS order.  Child 0 is the reference's low child (node i+1), child 1 its high
-// child (node i + high_child_offset).  A child reference with bit 31 set names
-// a leaf (index into the leaf table), otherwise an interior record.
+// child (node i + high_child_offset).  A child reference with bit 31 set is a
+// leaf and holds the index of its first triangle in the leaf-ordered triangle
+// array (the reference's leaf_ordered_indices order); the leaf's face count (the
+// reference's u16 num_faces) sits in `counts`.  Otherwise the reference is the
+// index of the child's interior record.
 //   f[0..5]   child 0 box p0.xyz, p1.xyz
 //   f[6..11]  child 1 box p0.xyz, p1.xyz
-//   u[12]     child 0 ref, u[13] child 1 ref, u[14] split axis, u[15] unused
+//   u[12]     child 0 ref, u[13] child 1 ref
+//   u[14]     child 0 count | child 1 count << 16   (0 for an interior child)
+//   u[15]     split axis
 struct alignas(64) Node64 {
    float box[12];
    uint32_t ref[2];
+   uint32_t counts;
    uint32_t axis;
-   uint32_t pad;
 };
 static_assert(sizeof(Node64) == 64, "Node64 is 64 bytes");
 
 constexpr uint32_t kLeafBit = 0x80000000u;
-
-// Leaf table entry: first triangle (index into the leaf-ordered triangle
-// array, which is the reference's leaf_ordered_indices order) and the count
-// (the reference's u16 num_faces).
-struct LeafRec {
-   uint32_t first, count;
-};
 
 // Per-mesh shading record.
 struct MeshRec {
@@ -49,13 +47,13 @@ struct MeshRec {
 
 struct DevScene {
    const Node64 *nodes;
-   const LeafRec *leaves;
    const float *tris;        // 9 floats (world v0, v1, v2) per leaf-ordered triangle
    const uint32_t *pairs;    // (mesh, face) per leaf-ordered triangle
    const MeshRec *meshes;
    const float *normals;     // world corner normals, 9 floats per leaf-ordered triangle (or null)
    float root_box[6];
-   uint32_t root_ref;        // kLeafBit|leaf or interior index
+   uint32_t root_ref;        // kLeafBit|first triangle, or interior index
+   uint32_t root_count;      // face count when the root is a leaf
    uint32_t num_tris;
    uint32_t max_stack;       // deepest deferred-stack use + 1 (tree depth bound)
    int32_t has_sky;

@@ -6,8 +6,8 @@
 // test re-transform its vertices.  Here the host transforms each face's
 // vertices once, with the reference's ApplyInPlace arithmetic, and lays the
 // scene out for the traversal kernel:
-//   * 64-byte interior records holding both child boxes (Node64),
-//   * a leaf table (first triangle, count),
+//   * 64-byte interior records holding both child boxes, child references and
+//     leaf face counts (Node64),
 //   * world-space triangles (36 B) in the reference's leaf order, so a leaf's
 //     faces are contiguous,
 //   * the (mesh, face) id of every leaf-ordered triangle, read once per ray.
@@ -85,21 +85,20 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
    std::vector<uint32_t> ref(nn);
    uint32_t n_int = 0, n_leaf = 0;
    for (size_t i = 0; i < nn; i++) {
-      if (bvh.nodes[i].num_faces > 0) ref[i] = kLeafBit | n_leaf++;
-      else ref[i] = n_int++;
-   }
-   img.nodes.assign(n_int, Node64{});
-   img.leaves.assign(n_leaf, LeafRec{});
-   for (size_t i = 0; i < nn; i++) {
       const ptx_bvh_node &b = bvh.nodes[i];
       if (b.num_faces > 0) {
-         LeafRec &l = img.leaves[ref[i] & ~kLeafBit];
-         l.first = b.offset;
-         l.count = b.num_faces;
-         if (size_t(b.offset) + b.num_faces > np)
+         if (size_t(b.offset) + b.num_faces > np || b.offset >= kLeafBit)
             return set_error(PTX_ERR_INVALID, "BVH leaf range exceeds the leaf pair array");
-         continue;
+         ref[i] = kLeafBit | b.offset;
+         n_leaf++;
+      } else {
+         ref[i] = n_int++;
       }
+   }
+   img.nodes.assign(n_int, Node64{});
+   for (size_t i = 0; i < nn; i++) {
+      const ptx_bvh_node &b = bvh.nodes[i];
+      if (b.num_faces > 0) continue;
       const size_t lo = i + 1, hi = i + b.offset;
       if (lo >= nn || hi >= nn || b.offset < 2 || b.flags > 2)
          return set_error(PTX_ERR_INVALID, "BVH interior node has an out-of-range child");
@@ -111,8 +110,8 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
       }
       r.ref[0] = ref[lo];
       r.ref[1] = ref[hi];
+      r.counts = (uint32_t)c0.num_faces | ((uint32_t)c1.num_faces << 16);
       r.axis = b.flags;
-      r.pad = 0;
    }
    // deepest chain of interior nodes bounds the deferred stack
    {
@@ -130,6 +129,7 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
       img.max_stack = depth + 1;
    }
    img.root_ref = ref[0];
+   img.root_count = bvh.nodes[0].num_faces;
    for (int k = 0; k < 3; k++) {
       img.root_box[k] = bvh.nodes[0].p0[k];
       img.root_box[3 + k] = bvh.nodes[0].p1[k];

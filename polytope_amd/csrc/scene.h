@@ -23,12 +23,11 @@ namespace ptx {
 
 struct HostImage {
    std::vector<Node64> nodes;
-   std::vector<LeafRec> leaves;
    std::vector<float> tris;
    std::vector<float> normals;
    std::vector<MeshRec> meshes;
    float root_box[6] = {0, 0, 0, 0, 0, 0};
-   uint32_t root_ref = 0;
+   uint32_t root_ref = 0, root_count = 0;
    uint32_t max_stack = 1;
    uint32_t num_interior = 0, num_leaves = 0;
 };

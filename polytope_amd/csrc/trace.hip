@@ -16,35 +16,62 @@
 namespace ptx {
 namespace dev {
 
-struct StackView {
-   uint2 *lds;         // [kLds][blockDim]
-   uint2 *spill;       // [max_stack - kLds][stride]
-   uint32_t tid, gtid, stride, lds_depth, max_stack;
-   __device__ __forceinline__ void put(uint32_t k, uint2 e) const {
-      if (k < lds_depth) lds[k * blockDim.x + tid] = e;
-      else spill[(size_t)(k - lds_depth) * stride + gtid] = e;
+// Deferred-child stack of one lane: entries [0, kLds) in LDS, deeper ones in a
+// global spill array.  The LDS part is three arrays (reference, slab near
+// distance, leaf face count) indexed [entry][thread] so a wave's accesses hit
+// consecutive banks; spill entries go through non-temporal global accesses
+// (and so never share an instruction with the LDS path).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int kLds>
+struct Stack {
+   uint32_t *ref;      // __shared__ [kLds * 256]
+   float *near;        // __shared__ [kLds * 256]
+   uint16_t *cnt;      // __shared__ [kLds * 256]
+   u32x4 *spill;       // [max_stack - kLds][stride]
+   uint32_t tid, gtid, stride, max_stack;
+   __device__ __forceinline__ void put(uint32_t k, uint32_t r, float nr, uint32_t c) const {
+      if (k < (uint32_t)kLds) {
+         ref[k * 256 + tid] = r;
+         near[k * 256 + tid] = nr;
+         cnt[k * 256 + tid] = (uint16_t)c;
+      } else if (k < max_stack) {
+         const u32x4 e = {r, __float_as_uint(nr), c, 0u};
+         __builtin_nontemporal_store(e, &spill[(size_t)(k - kLds) * stride + gtid]);
+      }
    }
-   __device__ __forceinline__ uint2 get(uint32_t k) const {
-      if (k >= max_stack) return make_uint2(0xFFFFFFFFu, 0x7F800000u);   // unreachable: depth bounds sp
-      if (k < lds_depth) return lds[k * blockDim.x + tid];
-      return spill[(size_t)(k - lds_depth) * stride + gtid];
+   __device__ __forceinline__ void get(uint32_t k, uint32_t &r, float &nr, uint32_t &c) const {
+      if (k < (uint32_t)kLds) {
+         r = ref[k * 256 + tid];
+         nr = near[k * 256 + tid];
+         c = cnt[k * 256 + tid];
+      } else if (k < max_stack) {
+         const u32x4 e = __builtin_nontemporal_load(&spill[(size_t)(k - kLds) * stride + gtid]);
+         r = e.x; nr = __uint_as_float(e.y); c = e.z;
+      } else {                     // unreachable: the tree depth bounds the stack
+         r = 0xFFFFFFFFu; nr = kInf; c = 0;
+      }
    }
 };
 
 // Pops deferred children until one passes its pop-time test; returns false
 // when the stack is empty (traversal done).
-__device__ __forceinline__ bool pop_next(Trav &s, const StackView &st) {
+template <int kLds>
+__device__ __forceinline__ bool pop_next(Trav &s, const Stack<kLds> &st) {
    while (s.sp > 0) {
       s.sp--;
-      const uint2 e = st.get(s.sp);
-      const float near = __uint_as_float(e.y);
-      if (near < 0.f || near <= s.min_t) {
+      uint32_t r, c;
+      float nr;
+      st.get(s.sp, r, nr, c);
+      if (nr < 0.f || nr <= s.min_t) {
          s.bb_hits++;
-         s.cur = e.x;
+         s.cur = r;
+         s.cnt = c;
          return true;
       }
       s.bb_misses++;
    }
+   s.cur = 0xFFFFFFFFu;
    return false;
 }
 
@@ -62,33 +89,62 @@ __device__ __forceinline__ void begin(Trav &s, const DevScene &sc) {
    if (inside || (near <= far && near <= s.min_t)) {
       s.bb_hits++;
       s.cur = sc.root_ref;
+      s.cnt = sc.root_count;
    } else {
       s.bb_misses++;
       s.cur = 0xFFFFFFFFu;
+      s.cnt = 0;
    }
 }
 
-// One step: visit the interior node or leaf in s.cur.  Returns false when the
-// traversal has finished.
-__device__ __forceinline__ bool step(Trav &s, const DevScene &sc, const StackView &st, bool debug_pixel) {
+// One step: visit the interior node in s.cur, or test up to two faces of the
+// leaf in s.cur.  Loads for either kind are issued before any arithmetic so a
+// wave whose lanes are split between nodes and leaves waits once per step.
+// Returns false when the traversal has finished.
+template <int kLds>
+__device__ __forceinline__ bool step(Trav &s, const DevScene &sc, const Stack<kLds> &st, bool debug_pixel) {
    if (s.cur == 0xFFFFFFFFu) return false;
-   if (s.cur & kLeafBit) {
-      const LeafRec lf = sc.leaves[s.cur & ~kLeafBit];
-      for (uint32_t i = 0; i < lf.count; i++) {
-         const uint32_t t = lf.first + i;
-         if (test_triangle(sc.tris + 9 * (size_t)t, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
-            s.hit = t;
+   const bool leaf = (s.cur & kLeafBit) != 0;
+   float4 qa, qb, qc;
+   uint4 qm;
+   float ta[9], tb[9];
+   const uint32_t first = s.cur & ~kLeafBit;
+   const bool two = leaf && s.cnt > 1;
+   if (!leaf) {
+      const float4 *rec = reinterpret_cast<const float4 *>(sc.nodes + s.cur);
+      qa = rec[0]; qb = rec[1]; qc = rec[2];
+      qm = reinterpret_cast<const uint4 *>(rec)[3];
+   } else {
+      const float *p = sc.tris + 9 * (size_t)first;
+#pragma unroll
+      for (int k = 0; k < 9; k++) ta[k] = p[k];
+      if (two) {
+#pragma unroll
+         for (int k = 0; k < 9; k++) tb[k] = p[9 + k];
+      }
+   }
+   if (leaf) {
+      if (test_triangle(ta, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
+         s.hit = first;
+         s.after = 0;
+      }
+      s.after++;
+      if (two) {
+         if (test_triangle(tb, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
+            s.hit = first + 1;
             s.after = 0;
          }
          s.after++;
       }
-      s.tris += lf.count;
-      if (!pop_next(s, st)) { s.cur = 0xFFFFFFFFu; return false; }
-      return true;
+      const uint32_t done = two ? 2u : 1u;
+      s.tris += done;
+      if (s.cnt > done) {            // more faces in this leaf: continue it next step
+         s.cur += done;
+         s.cnt -= done;
+         return true;
+      }
+      return pop_next(s, st);
    }
-   const float4 *rec = reinterpret_cast<const float4 *>(sc.nodes + s.cur);
-   const float4 qa = rec[0], qb = rec[1], qc = rec[2];
-   const uint4 qm = reinterpret_cast<const uint4 *>(rec)[3];
    bool in0, in1;
    float n0, f0, n1, f1;
    slab(qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, s.r, in0, n0, f0);
@@ -96,30 +152,33 @@ __device__ __forceinline__ bool step(Trav &s, const DevScene &sc, const StackVie
    const bool h0 = in0 || (n0 <= f0 && n0 <= s.min_t);
    const bool h1 = in1 || (n1 <= f1 && n1 <= s.min_t);
    // bvh.cpp:510-517: with neg_dir[axis] the high child (child 1) is popped first.
-   const bool high_first = (s.r.neg >> qm.z) & 1u;
+   const bool high_first = (s.r.neg >> qm.w) & 1u;
    const bool fh = high_first ? h1 : h0, sh = high_first ? h0 : h1;
+   const uint32_t c0 = qm.z & 0xFFFFu, c1 = qm.z >> 16;
    const uint32_t fref = high_first ? qm.y : qm.x, sref = high_first ? qm.x : qm.y;
+   const uint32_t fcnt = high_first ? c1 : c0, scnt = high_first ? c0 : c1;
    const float snear = high_first ? (in0 ? -1.f : n0) : (in1 ? -1.f : n1);
    if (fh) {
       s.bb_hits++;
       if (sh) {
-         if (s.sp < st.max_stack) st.put(s.sp, make_uint2(sref, __float_as_uint(snear)));
+         st.put(s.sp, sref, snear, scnt);
          s.sp++;
       } else {
          s.bb_misses++;
       }
       s.cur = fref;
+      s.cnt = fcnt;
       return true;
    }
    s.bb_misses++;
    if (sh) {
       s.bb_hits++;
       s.cur = sref;
+      s.cnt = scnt;
       return true;
    }
    s.bb_misses++;
-   if (!pop_next(s, st)) { s.cur = 0xFFFFFFFFu; return false; }
-   return true;
+   return pop_next(s, st);
 }
 
 __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, uint32_t id, ptx_hit *hits,
@@ -156,12 +215,14 @@ constexpr int kSegments = 8;
 template <int kLds>
 __global__ void __launch_bounds__(256) k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n,
                                                      ptx_hit *__restrict__ hits, ptx_ray_counters *counters,
-                                                     ptx_hit_frame *frames, uint32_t *work, uint2 *spill,
+                                                     ptx_hit_frame *frames, uint32_t *work, uint4 *spill,
                                                      uint32_t refill_min) {
-   __shared__ uint2 lds[kLds * 256];
+   __shared__ uint32_t lds_ref[kLds * 256];
+   __shared__ float lds_near[kLds * 256];
+   __shared__ uint16_t lds_cnt[kLds * 256];
    const uint32_t lane = __lane_id();
-   const StackView st{lds, spill, threadIdx.x, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x,
-                      (uint32_t)kLds, sc.max_stack};
+   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x, blockIdx.x * blockDim.x + threadIdx.x,
+                        gridDim.x * blockDim.x, sc.max_stack};
    const uint32_t xcd = blockIdx.x % kSegments;
    Trav s;
    bool active = false;
@@ -200,7 +261,7 @@ __global__ void __launch_bounds__(256) k_closest_hit(DevScene sc, const float *_
       if (active) {
          // run until this lane finishes or enough lanes of the wave are idle
          while (true) {
-            if (!step(s, sc, st, false)) {
+            if (!step<kLds>(s, sc, st, false)) {
                write_result(s, sc, id, hits, counters, frames);
                active = false;
             }
@@ -222,7 +283,7 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 // Host launcher.  work: >= 8 u32 of device memory; spill sized by the caller
 // for (max_stack - lds) * grid * 256 entries.
 int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits,
-                       ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint2 *d_spill,
+                       ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
                        uint32_t grid, uint32_t lds_depth, uint32_t refill_min, hipStream_t stream) {
    hipLaunchKernelGGL(dev::k_zero_u32, dim3(1), dim3(64), 0, stream, d_work, (uint32_t)dev::kSegments);
    if (n == 0) return 0;

@@ -120,7 +120,7 @@ __device__ __forceinline__ float max_component(float x, float y, float z) {   //
 
 // Mesh::intersect test part for one face (mesh.cpp:196-340).  On acceptance
 // updates min_t and the hit's (u, v, w) and returns true.
-__device__ __forceinline__ bool test_triangle(const float *__restrict__ v, const Ray &r, float &min_t,
+__device__ __forceinline__ bool test_triangle(const float *v, const Ray &r, float &min_t,
                                               float &hu, float &hv, float &hw, bool debug_pixel) {
    if (debug_pixel && r.dx != r.dx) return false;              // mesh.cpp:192-194
    const float v0x = v[0], v0y = v[1], v0z = v[2];
@@ -272,7 +272,8 @@ struct Trav {
    float min_t, hu, hv, hw;
    uint32_t hit;          // leaf-ordered triangle index, 0xFFFFFFFF = none
    uint32_t after;        // face tests since (and including) the accepting one
-   uint32_t cur;          // node to process next (kLeafBit | leaf, or interior index)
+   uint32_t cur;          // node to process next: kLeafBit | first triangle, or interior index
+   uint32_t cnt;          // faces left in the leaf `cur` (0 for an interior node)
    uint32_t sp;           // deferred stack depth
    uint32_t bb_hits, bb_misses, tris;
 };

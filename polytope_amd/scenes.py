@@ -390,7 +390,7 @@ def closest_hit_workload(subdiv: int, n_rays: int, seed: int = 2):
     n_coh = n_rays // 2
     nx = int(round(math.sqrt(n_coh * 2)))
     ny = max(1, n_coh // nx)
-    cam = Camera(look_at((0.0, 0.4, -3.2), (0.0, 0.0, 0.0)), 60.0, nx, ny)
+    cam = Camera(look_at((0.0, 0.3, -2.5), (0.0, 0.0, 0.0)), 44.0, nx, ny)
     coh = rays_coherent(cam, nx, ny)
     lo, hi = scene_bounds(meshes)
     inc = rays_incoherent(lo, hi, n_rays - coh.shape[0], seed + 100)

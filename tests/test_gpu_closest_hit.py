@@ -200,7 +200,7 @@ def test_gpu_full_size_closest_hit_sample(dev):
     s = h["u"][hit].astype(np.float64) + h["v"][hit] + h["w"][hit]
     assert np.abs(s - 1).max() < 1e-5
     assert ((c["bb_hits"] + c["bb_misses"]) % 2 == 1).all()
-    assert hit[:n_coh].mean() > 0.3
+    assert hit[:n_coh].mean() > 0.35       # the coherent grid is framed on the sphere
     idx = np.arange(0, len(rays), 61)
     ref = O.OracleScene(meshes).intersect(rays[idx])
     assert_same_hits(h[idx], c[idx], None, ref, frames=False)
