@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--lds", type=int, default=0, help="LDS stack entries per lane (0 = library default)")
+    ap.add_argument("--refill", type=int, default=0, help="idle lanes that trigger a refill (0 = default)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-coherent", action="store_true", help="skip the coherent-rate measurement")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (profiler calibration)")
@@ -99,8 +101,8 @@ def main():
     print(f"[bench] workload built: {meshes[0].num_faces} triangles, {n_rays} rays "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     dev = Device(0)
-    if args.lds:
-        dev.set_tuning(lds_stack=args.lds)
+    if args.lds or args.refill or args.blocks_per_cu:
+        dev.set_tuning(lds_stack=args.lds, refill_lanes=args.refill, blocks_per_cu=args.blocks_per_cu)
     sc = Scene(dev, meshes)
     info = sc.info()
     print(f"[bench] scene: {info['num_nodes']} nodes, build {info['build_seconds']:.2f} s, "

@@ -173,9 +173,11 @@ int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_
 
 /* Tuning knobs shared by ptx_render_batch and the server; zero = autotuned. */
 typedef struct ptx_tuning {
-   uint32_t paths_per_wave;  /* path slots in the wavefront queues         */
-   uint32_t block_size;      /* threads per block of the traversal kernel  */
-   uint32_t lds_stack;       /* LDS stack entries per lane                 */
+   uint32_t paths_per_wave;  /* path slots in the wavefront queues                    */
+   uint32_t block_size;      /* threads per block of the traversal kernel (256)       */
+   uint32_t lds_stack;       /* LDS stack entries per lane: 8, 12, 16 or 24           */
+   uint32_t refill_lanes;    /* idle lanes that trigger a wave's ray refill (1..64)   */
+   uint32_t blocks_per_cu;   /* persistent blocks per CU; 0 = occupancy limit         */
 } ptx_tuning;
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);

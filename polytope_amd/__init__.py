@@ -108,14 +108,15 @@ class Device:
             load().ptx_device_close(self.handle)
             self.handle = None
 
-    def set_tuning(self, paths_per_wave: int = 0, block_size: int = 0, lds_stack: int = 0) -> None:
-        t = _abi.Tuning(paths_per_wave, block_size, lds_stack)
+    def set_tuning(self, paths_per_wave: int = 0, block_size: int = 0, lds_stack: int = 0, refill_lanes: int = 0,
+                   blocks_per_cu: int = 0) -> None:
+        t = _abi.Tuning(paths_per_wave, block_size, lds_stack, refill_lanes, blocks_per_cu)
         check(load().ptx_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _abi.Tuning()
         check(load().ptx_get_tuning(self.handle, C.byref(t)))
-        return {"paths_per_wave": t.paths_per_wave, "block_size": t.block_size, "lds_stack": t.lds_stack}
+        return {k: getattr(t, k) for k, _ in _abi.Tuning._fields_}
 
     def __del__(self):
         try:

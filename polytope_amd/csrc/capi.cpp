@@ -47,7 +47,9 @@ int upload(ptx_scene *s, const std::vector<T> &v, const T **dst) {
 uint32_t lds_depth_of(const ptx_device *dev) { return dev->tuning.lds_stack ? dev->tuning.lds_stack : 12; }
 
 uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds) {
-   return (uint32_t)(dev->num_cus * closest_hit_blocks_per_cu(lds));
+   int per_cu = closest_hit_blocks_per_cu(lds);
+   if (dev->tuning.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)dev->tuning.blocks_per_cu);
+   return (uint32_t)(dev->num_cus * per_cu);
 }
 
 size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds) {
@@ -67,7 +69,7 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
    PTX_HIP_TRY(hipMallocAsync(&scratch, closest_hit_scratch_bytes(s, grid, lds), stream));
    uint32_t *work = static_cast<uint32_t *>(scratch);
    uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(scratch) + 256);
-   const uint32_t refill = dev->tuning.block_size ? 16 : 16;
+   const uint32_t refill = dev->tuning.refill_lanes ? std::min(64u, dev->tuning.refill_lanes) : 1u;
    const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill,
                                      stream);
    const hipError_t fe = hipFreeAsync(scratch, stream);
@@ -267,6 +269,7 @@ int ptx_get_tuning(ptx_device *dev, ptx_tuning *t) {
    *t = dev->tuning;
    if (!t->lds_stack) t->lds_stack = lds_depth_of(dev);
    if (!t->block_size) t->block_size = 256;
+   if (!t->refill_lanes) t->refill_lanes = 1;
    return PTX_OK;
 }
 

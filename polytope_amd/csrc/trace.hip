@@ -97,40 +97,47 @@ __device__ __forceinline__ void begin(Trav &s, const DevScene &sc) {
    }
 }
 
-// One step: visit the interior node in s.cur, or test up to two faces of the
-// leaf in s.cur.  Loads for either kind are issued before any arithmetic so a
-// wave whose lanes are split between nodes and leaves waits once per step.
-// Returns false when the traversal has finished.
-template <int kLds>
-__device__ __forceinline__ bool step(Trav &s, const DevScene &sc, const Stack<kLds> &st, bool debug_pixel) {
-   if (s.cur == 0xFFFFFFFFu) return false;
-   const bool leaf = (s.cur & kLeafBit) != 0;
-   float4 qa, qb, qc;
-   uint4 qm;
-   float ta[9], tb[9];
-   const uint32_t first = s.cur & ~kLeafBit;
-   const bool two = leaf && s.cnt > 1;
-   if (!leaf) {
-      const float4 *rec = reinterpret_cast<const float4 *>(sc.nodes + s.cur);
-      qa = rec[0]; qb = rec[1]; qc = rec[2];
-      qm = reinterpret_cast<const uint4 *>(rec)[3];
-   } else {
-      const float *p = sc.tris + 9 * (size_t)first;
+// Memory operands of one traversal step, in one block of registers whatever
+// their kind: an interior record (q[0..11] child boxes, q[12..15] refs, counts,
+// axis), up to two leaf faces (q[0..8], q[9..17]) or a new ray (q[0..5]).
+struct Fetched {
+   float q[18];
+};
+
+__device__ __forceinline__ void fetch(const Trav &s, const DevScene &sc, Fetched &f) {
+   if (s.cur & kLeafBit) {
+      const float *p = sc.tris + 9 * (size_t)(s.cur & ~kLeafBit);
 #pragma unroll
-      for (int k = 0; k < 9; k++) ta[k] = p[k];
-      if (two) {
+      for (int k = 0; k < 9; k++) f.q[k] = p[k];
+      if (s.cnt > 1) {
 #pragma unroll
-         for (int k = 0; k < 9; k++) tb[k] = p[9 + k];
+         for (int k = 0; k < 9; k++) f.q[9 + k] = p[9 + k];
       }
+   } else {
+      const float4 *rec = reinterpret_cast<const float4 *>(sc.nodes + s.cur);
+      const float4 a = rec[0], b = rec[1], c = rec[2], m = rec[3];
+      f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
+      f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
+      f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
+      f.q[12] = m.x; f.q[13] = m.y; f.q[14] = m.z; f.q[15] = m.w;
    }
-   if (leaf) {
-      if (test_triangle(ta, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
+}
+
+// One step on fetched operands: test up to two faces of the leaf s.cur, or the
+// two children of the interior node s.cur.  Returns false when the traversal
+// has finished.
+template <int kLds>
+__device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<kLds> &st, bool debug_pixel) {
+   if (s.cur & kLeafBit) {
+      const uint32_t first = s.cur & ~kLeafBit;
+      const bool two = s.cnt > 1;
+      if (test_triangle(f.q, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
          s.hit = first;
          s.after = 0;
       }
       s.after++;
       if (two) {
-         if (test_triangle(tb, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
+         if (test_triangle(f.q + 9, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
             s.hit = first + 1;
             s.after = 0;
          }
@@ -147,15 +154,17 @@ __device__ __forceinline__ bool step(Trav &s, const DevScene &sc, const Stack<kL
    }
    bool in0, in1;
    float n0, f0, n1, f1;
-   slab(qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, s.r, in0, n0, f0);
-   slab(qb.z, qb.w, qc.x, qc.y, qc.z, qc.w, s.r, in1, n1, f1);
+   slab(f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5], s.r, in0, n0, f0);
+   slab(f.q[6], f.q[7], f.q[8], f.q[9], f.q[10], f.q[11], s.r, in1, n1, f1);
+   const uint32_t mref0 = __float_as_uint(f.q[12]), mref1 = __float_as_uint(f.q[13]);
+   const uint32_t mcounts = __float_as_uint(f.q[14]), maxis = __float_as_uint(f.q[15]);
    const bool h0 = in0 || (n0 <= f0 && n0 <= s.min_t);
    const bool h1 = in1 || (n1 <= f1 && n1 <= s.min_t);
    // bvh.cpp:510-517: with neg_dir[axis] the high child (child 1) is popped first.
-   const bool high_first = (s.r.neg >> qm.w) & 1u;
+   const bool high_first = (s.r.neg >> maxis) & 1u;
    const bool fh = high_first ? h1 : h0, sh = high_first ? h0 : h1;
-   const uint32_t c0 = qm.z & 0xFFFFu, c1 = qm.z >> 16;
-   const uint32_t fref = high_first ? qm.y : qm.x, sref = high_first ? qm.x : qm.y;
+   const uint32_t c0 = mcounts & 0xFFFFu, c1 = mcounts >> 16;
+   const uint32_t fref = high_first ? mref1 : mref0, sref = high_first ? mref0 : mref1;
    const uint32_t fcnt = high_first ? c1 : c0, scnt = high_first ? c0 : c1;
    const float snear = high_first ? (in0 ? -1.f : n0) : (in1 ? -1.f : n1);
    if (fh) {
@@ -211,9 +220,43 @@ __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, 
 }
 
 constexpr int kSegments = 8;
+constexpr uint32_t kBatch = 64;     // ray ids taken per atomic
+
+// Per-wave source of ray ids: eight segments of the ray list, one per XCD
+// (blockIdx % 8 is the XCD a block runs on), each drained by an atomic counter
+// in batches of 64.  Lane 0 requests the next batch while the current one is
+// handed out, so the atomic's latency overlaps traversal steps.
+struct RaySource {
+   uint32_t n, seg, tries, pool_base, pool_left, prefetch;   // wave-uniform except prefetch (lane 0)
+   bool done;
+   __device__ __forceinline__ uint32_t seg_lo(uint32_t sg) const { return (uint32_t)((uint64_t)n * sg / kSegments); }
+   __device__ __forceinline__ uint32_t seg_size(uint32_t sg) const { return seg_lo(sg + 1) - seg_lo(sg); }
+   __device__ __forceinline__ void request(uint32_t *work, uint32_t lane) {
+      if (lane == 0) prefetch = atomicAdd(&work[seg], kBatch);
+   }
+   // Makes pool_left > 0 unless every segment is drained (then done = true).
+   __device__ __forceinline__ void refill(uint32_t *work, uint32_t lane) {
+      while (pool_left == 0 && !done) {
+         const uint32_t b = __builtin_amdgcn_readlane(prefetch, 0);
+         const uint32_t size = seg_size(seg);
+         if (b < size) {
+            pool_base = seg_lo(seg) + b;
+            pool_left = min(kBatch, size - b);
+            request(work, lane);
+         } else if (++tries < kSegments) {
+            seg = (seg + 1) % kSegments;
+            request(work, lane);
+         } else {
+            done = true;
+         }
+      }
+   }
+};
+
+enum : uint32_t { kIdle = 0, kPending = 1, kActive = 2 };
 
 template <int kLds>
-__global__ void __launch_bounds__(256) k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n,
                                                      ptx_hit *__restrict__ hits, ptx_ray_counters *counters,
                                                      ptx_hit_frame *frames, uint32_t *work, uint4 *spill,
                                                      uint32_t refill_min) {
@@ -221,53 +264,55 @@ __global__ void __launch_bounds__(256) k_closest_hit(DevScene sc, const float *_
    __shared__ float lds_near[kLds * 256];
    __shared__ uint16_t lds_cnt[kLds * 256];
    const uint32_t lane = __lane_id();
-   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x, blockIdx.x * blockDim.x + threadIdx.x,
-                        gridDim.x * blockDim.x, sc.max_stack};
-   const uint32_t xcd = blockIdx.x % kSegments;
+   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
+                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, sc.max_stack};
+   RaySource src{n, blockIdx.x % kSegments, 0u, 0u, 0u, 0u, false};
+   src.request(work, lane);
    Trav s;
-   bool active = false;
-   bool exhausted = false;
+   uint32_t mode = kIdle;
    uint32_t id = 0;
    while (true) {
-      const uint64_t idle = __ballot(!active);
-      const uint32_t n_idle = __popcll(idle);
-      if (!exhausted && n_idle >= (idle == __ballot(1) ? 1u : refill_min)) {
-         uint32_t base = 0, got = 0;
-         if (lane == 0) {
-            for (int a = 0; a < kSegments && got == 0; a++) {
-               const uint32_t seg = (xcd + a) % kSegments;
-               const uint32_t lo = (uint32_t)((uint64_t)n * seg / kSegments);
-               const uint32_t hi = (uint32_t)((uint64_t)n * (seg + 1) / kSegments);
-               if (__hip_atomic_load(&work[seg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi - lo) continue;
-               const uint32_t b = atomicAdd(&work[seg], n_idle);
-               if (b < hi - lo) { base = lo + b; got = min(n_idle, hi - lo - b); }
+      // hand out ray ids to idle lanes (no memory access unless a batch is used up)
+      uint64_t idle = __ballot(mode == kIdle);
+      if (idle && !src.done && (uint32_t)__popcll(idle) >= refill_min) {
+         while (idle) {
+            src.refill(work, lane);
+            if (src.done) break;
+            const uint32_t take = min((uint32_t)__popcll(idle), src.pool_left);
+            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+            const bool mine = ((idle >> lane) & 1ull) && rank < take;
+            if (mine) {
+               id = src.pool_base + rank;
+               mode = kPending;
             }
-         }
-         base = __shfl(base, 0);
-         got = __shfl(got, 0);
-         if (got == 0) exhausted = true;
-         if (!active) {
-            const uint32_t slot = __popcll(idle & ((1ull << lane) - 1ull));
-            if (slot < got) {
-               id = base + slot;
-               const float *rp = rays + 6 * (size_t)id;
-               make_ray(s.r, rp[0], rp[1], rp[2], rp[3], rp[4], rp[5]);
-               begin(s, sc);
-               active = true;
-            }
+            idle &= ~__ballot(mine);
+            src.pool_base += take;
+            src.pool_left -= take;
          }
       }
-      if (!__any(active)) break;
-      if (active) {
-         // run until this lane finishes or enough lanes of the wave are idle
-         while (true) {
-            if (!step<kLds>(s, sc, st, false)) {
-               write_result(s, sc, id, hits, counters, frames);
-               active = false;
-            }
-            // lanes outside this loop are idle; count the ones still active in it
-            const uint32_t idle_now = 64u - (uint32_t)__popcll(__ballot(active));
-            if (!active || (!exhausted && idle_now >= refill_min)) break;
+      if (!__any(mode != kIdle)) break;
+      // fetch: the rays of newly assigned lanes and every active lane's operands
+      Fetched f;
+      if (mode == kPending) {
+         const float *rp = rays + 6 * (size_t)id;
+#pragma unroll
+         for (int k = 0; k < 6; k++) f.q[k] = rp[k];
+      } else if (mode == kActive) {
+         fetch(s, sc, f);
+      }
+      // compute
+      if (mode == kPending) {
+         make_ray(s.r, f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5]);
+         begin(s, sc);
+         mode = kActive;
+         if (s.cur == 0xFFFFFFFFu) {
+            write_result(s, sc, id, hits, counters, frames);
+            mode = kIdle;
+         }
+      } else if (mode == kActive) {
+         if (!advance<kLds>(s, f, st, false)) {
+            write_result(s, sc, id, hits, counters, frames);
+            mode = kIdle;
          }
       }
    }
