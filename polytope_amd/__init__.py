@@ -20,7 +20,7 @@ from . import _abi
 from ._abi import (BVH_NODE_DTYPE, COUNTER_DTYPE, FRAME_DTYPE, HIT_DTYPE, PtxError, check, load)
 from .scenes import Camera, MeshData
 
-__all__ = ["Device", "Scene", "PtxError", "HIT_DTYPE", "COUNTER_DTYPE", "FRAME_DTYPE", "BVH_NODE_DTYPE",
+__all__ = ["Device", "Scene", "PtxError", "render", "render_batch", "HIT_DTYPE", "COUNTER_DTYPE", "FRAME_DTYPE", "BVH_NODE_DTYPE",
            "mesh_descs", "build_bvh", "camera_desc", "job_desc", "MeshData", "Camera"]
 
 
@@ -197,3 +197,24 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def render(scene: Scene, cam: Camera, spp: int, max_depth: int, seed: int) -> np.ndarray:
+    """Plain GPU render of one job (ptx_render): one launch per sample, one thread
+    per pixel path.  Returns the (H, W, 3) BoxFilter film (0-255 units)."""
+    film = np.zeros((cam.height, cam.width, 3), np.float32)
+    job = job_desc(cam, spp, max_depth, seed)
+    check(load().ptx_render(scene.handle, C.byref(job), film.ctypes.data))
+    return film
+
+
+def render_batch(jobs: Sequence[Tuple[Scene, Camera, int, int, int]]) -> List[np.ndarray]:
+    """Batched wavefront render (ptx_render_batch) of (scene, camera, spp,
+    max_depth, seed) jobs in one call; returns one film per job."""
+    n = len(jobs)
+    handles = (C.c_void_p * n)(*[j[0].handle.value for j in jobs])
+    descs = (_abi.JobDesc * n)(*[job_desc(j[1], j[2], j[3], j[4]) for j in jobs])
+    films = [np.zeros((j[1].height, j[1].width, 3), np.float32) for j in jobs]
+    ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in films])
+    check(load().ptx_render_batch(handles, descs, n, ptrs))
+    return films

@@ -8,7 +8,7 @@
 #include <string>
 #include <thread>
 
-#include "scene.h"
+#include "render_host.h"
 
 namespace ptx {
 
@@ -80,6 +80,8 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
 
 }  // namespace ptx
 
+ptx_device::~ptx_device() = default;
+
 using namespace ptx;
 
 extern "C" {
@@ -117,7 +119,9 @@ int ptx_device_open(int ordinal, ptx_device **out) {
 void ptx_device_close(ptx_device *dev) {
    if (!dev) return;
    (void)hipSetDevice(dev->ordinal);
+   dev->engine.reset();
    if (dev->stream) (void)hipStreamDestroy(dev->stream);
+   dev->stream = nullptr;
    delete dev;
 }
 

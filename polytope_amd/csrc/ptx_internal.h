@@ -45,25 +45,38 @@ struct MeshRec {
    uint32_t normal_base;   // index of the mesh's first corner normal / 3 (per face)
 };
 
+// Device view of one scene.  The first 48 bytes are what a traversal needs
+// (loaded per lane by the render path's extend stage as three 16-byte loads).
 struct DevScene {
-   const Node64 *nodes;
+   const Node64 *nodes;      // interior records
    const float *tris;        // 9 floats (world v0, v1, v2) per leaf-ordered triangle
-   const uint32_t *pairs;    // (mesh, face) per leaf-ordered triangle
-   const MeshRec *meshes;
-   const float *normals;     // world corner normals, 9 floats per leaf-ordered triangle (or null)
    float root_box[6];
    uint32_t root_ref;        // kLeafBit|first triangle, or interior index
    uint32_t root_count;      // face count when the root is a leaf
+   const uint32_t *pairs;    // (mesh, face) per leaf-ordered triangle
+   const MeshRec *meshes;
+   const float *normals;     // world corner normals, 9 floats per leaf-ordered triangle (or null)
    uint32_t num_tris;
    uint32_t max_stack;       // deepest deferred-stack use + 1 (tree depth bound)
    int32_t has_sky;
    float sky[3];
 };
+static_assert(sizeof(DevScene) == 96, "DevScene layout");
 
-struct CameraParams {
-   float m[16];
+// Per-job constants of a render, device-resident in the job table.
+struct JobDev {
+   float cam[16];            // camera-to-world, row-major
    float oow, ooh, aspect, tanf2;
-   uint32_t width, height;
+   uint32_t width, height, spp, max_depth;
+   uint32_t seed, scene;     // RNG seed; index into the scene table
+   float *film;              // W*H*3 running sums (BoxFilter order), then the average
+   uint32_t *primary;        // W*H primary-hit records (16 words each)
+};
+
+// A contiguous block of path slots: samples [s0, s0 + k) of every pixel of a job.
+// Slot of (pixel p, sample s) = base + (s - s0) * npix + p.
+struct Chunk {
+   uint32_t job, s0, k, npix, base;
 };
 
 // ---------------------------------------------------------------- host-side scene

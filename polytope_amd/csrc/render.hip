@@ -1,0 +1,638 @@
+// Path-tracing kernels for gfx950: the bounce loop of
+// PathTraceIntegrator::GetSample (src/cpu/integrators/PathTraceIntegrator.cpp:10-135)
+// with CenterSampler pixel centres (src/cpu/samplers/samplers.cpp:13-18), the
+// pinhole camera (src/cpu/cameras/PerspectiveCamera.cpp:27-49), Lambert and mirror
+// BRDFs (src/cpu/shading/brdf/*.h) and BoxFilter accumulation
+// (src/cpu/filters/BoxFilter.cpp:14-71), in two forms:
+//
+//   plain     k_mega: one launch per sample, one thread per pixel path with every
+//             bounce inline — the launch shape of the reference CUDA path
+//             (PathTracerKernel::Trace, src/cuda/kernels/path_tracer.cu:899-1018);
+//   wavefront k_primary traces each pixel's primary ray once (CenterSampler gives
+//             every sample of a pixel the same ray), k_round_start shades the
+//             primary hit per sample, then per bounce k_extend (the persistent
+//             traversal of traverse.h fed from a queue) and k_shade (frame, BRDF,
+//             compaction into the next queue); k_accumulate adds each pixel's
+//             samples to its film in sample order and k_finalize scales by 1/spp.
+//
+// Both forms run the same arithmetic in the same order, so their films are
+// identical to each other and to the CPU restatement.
+#include <hip/hip_runtime.h>
+
+#include "traverse.h"
+
+#pragma clang fp contract(off)
+
+namespace ptx {
+namespace dev {
+
+constexpr float kPI = 3.14159265358979323846f;
+constexpr float kTwoPI = 2.0f * kPI;
+constexpr uint32_t kRngKeyHi = 0x706F6C79u;   // second Philox key word: ASCII "poly"
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC 2011): ten rounds of the
+// multiply-xor bijection with the Weyl key schedule.
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                              uint32_t k1, uint32_t &o0, uint32_t &o1) {
+#pragma unroll
+   for (int r = 0; r < 10; r++) {
+      const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+      const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+      const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+      c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+      k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+   }
+   o0 = c0;
+   o1 = c1;
+}
+
+__device__ __forceinline__ float u01(uint32_t r) { return (float)(r >> 8) * 0x1p-24f; }
+
+// sin and cos for theta in [0, 2pi]: quadrant k = floor(theta*2/pi + 1/2),
+// three-part Cody-Waite reduction, Taylor polynomials on [-pi/4, pi/4] in Horner
+// form with explicit fma (the same operation sequence as the CPU restatement).
+__device__ __forceinline__ void sincos_poly(float theta, float &s_out, float &c_out) {
+   const float q = theta * 0.636619772367581343f;
+   const int k = (int)__builtin_floorf(q + 0.5f);
+   const float kf = (float)k;
+   float r = fmaf_(-kf, 1.57079625129699707031f, theta);
+   r = fmaf_(-kf, 7.54978941586159635335e-08f, r);
+   r = fmaf_(-kf, 5.39030252995776476554e-15f, r);
+   const float r2 = r * r;
+   float ps = fmaf_(r2, 2.75573192239858906526e-06f, -1.98412698412698412698e-04f);
+   ps = fmaf_(r2, ps, 8.33333333333333333333e-03f);
+   ps = fmaf_(r2, ps, -1.66666666666666666667e-01f);
+   const float sn = fmaf_(r * r2, ps, r);
+   float pc = fmaf_(r2, -2.75573192239858906526e-07f, 2.48015873015873015873e-05f);
+   pc = fmaf_(r2, pc, -1.38888888888888888889e-03f);
+   pc = fmaf_(r2, pc, 4.16666666666666666667e-02f);
+   pc = fmaf_(r2, pc, -0.5f);
+   const float cs = fmaf_(r2, pc, 1.0f);
+   switch (k & 3) {
+      case 0: s_out = sn; c_out = cs; break;
+      case 1: s_out = cs; c_out = -sn; break;
+      case 2: s_out = -sn; c_out = -cs; break;
+      default: s_out = -cs; c_out = sn; break;
+   }
+}
+
+__device__ __forceinline__ void normalize3(float &x, float &y, float &z) {   // Vector::Normalize
+   const float ld = 1.0f / __builtin_sqrtf(x * x + y * y + z * z);
+   x *= ld; y *= ld; z *= ld;
+}
+
+// PerspectiveCamera::get_ray_for_pixel at the pixel centre (CenterSampler).
+__device__ __forceinline__ void camera_ray(const JobDev &J, uint32_t x, uint32_t y, float o[3], float d[3]) {
+   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+   const float nx = px * J.oow, ny = py * J.ooh;
+   const float cx = (2 * nx - 1) * J.aspect * J.tanf2;
+   const float cy = (1 - 2 * ny) * J.tanf2;
+   float dx = cx, dy = cy, dz = 1.0f;
+   normalize3(dx, dy, dz);
+   const float *m = J.cam;
+   // Transform::Apply(Point) of the origin and Apply(Vector) (Transform.cpp:93-133)
+   const float ox = 0.f * m[0] + 0.f * m[1] + 0.f * m[2] + m[3];
+   const float oy = 0.f * m[4] + 0.f * m[5] + 0.f * m[6] + m[7];
+   const float oz = 0.f * m[8] + 0.f * m[9] + 0.f * m[10] + m[11];
+   const float w = 0.f * m[12] + 0.f * m[13] + 0.f * m[14] + m[15];
+   if (w == 1) { o[0] = ox; o[1] = oy; o[2] = oz; }
+   else {
+      const float dv = 1.f / w;
+      o[0] = ox * dv; o[1] = oy * dv; o[2] = oz * dv;
+   }
+   float wx = dx * m[0] + dy * m[1] + dz * m[2];
+   float wy = dx * m[4] + dy * m[5] + dz * m[6];
+   float wz = dx * m[8] + dy * m[9] + dz * m[10];
+   normalize3(wx, wy, wz);
+   d[0] = wx; d[1] = wy; d[2] = wz;
+}
+
+struct Radiance {
+   float r, g, b;
+};
+
+// Terminal radiance of a path whose ray misses (PathTraceIntegrator.cpp:46-54).
+__device__ __forceinline__ Radiance miss_radiance(const DevScene &S, const float src[3]) {
+   Radiance L = {0.f, 0.f, 0.f};
+   if (S.has_sky) L = {S.sky[0] * src[0], S.sky[1] * src[1], S.sky[2] * src[2]};
+   L.r += 0.f; L.g += 0.f; L.b += 0.f;
+   return L;
+}
+
+// The bounce (PathTraceIntegrator.cpp:71-95): local incoming direction,
+// BRDF sample, world outgoing direction; updates src.
+__device__ __forceinline__ void bounce(const MeshRec &M, const Frame &fr, const float lin[3], uint32_t pixel,
+                                       uint32_t sample, uint32_t bounce_index, uint32_t seed, float wout[3],
+                                       float src[3]) {
+   float lx, ly, lz;
+   if (M.brdf == PTX_BRDF_MIRROR) {            // MirrorBRDF::sample (mirror_brdf.h:23-35)
+      lx = lin[0]; ly = -lin[1]; lz = lin[2];
+   } else {                                     // LambertBRDF -> AbstractBRDF::sample -> CosineSampleHemisphere
+      uint32_t r0, r1;
+      philox4x32_10(pixel, sample, bounce_index, 0u, seed, kRngKeyHi, r0, r1);
+      const float u0 = u01(r0), u1 = u01(r1);
+      const float r = __builtin_sqrtf(u0);
+      const float theta = kTwoPI * u1;
+      float sn, cs;
+      sincos_poly(theta, sn, cs);
+      const float om = 1.0f - u0;
+      lx = r * cs;
+      ly = __builtin_sqrtf(0.0f < om ? om : 0.0f);
+      lz = r * sn;
+   }
+   // Intersection::LocalToWorld (Intersection.cpp:16-23)
+   float wx = fr.t1x * lx + fr.nx * ly + fr.t2x * lz;
+   float wy = fr.t1y * lx + fr.ny * ly + fr.t2y * lz;
+   float wz = fr.t1z * lx + fr.nz * ly + fr.t2z * lz;
+   normalize3(wx, wy, wz);
+   wout[0] = wx; wout[1] = wy; wout[2] = wz;
+   src[0] = src[0] * M.refl[0];
+   src[1] = src[1] * M.refl[1];
+   src[2] = src[2] * M.refl[2];
+}
+
+// Intersection::WorldToLocal (Intersection.cpp:10-14)
+__device__ __forceinline__ void world_to_local(const Frame &fr, const float d[3], float lin[3]) {
+   float x = fr.t1x * d[0] + fr.t1y * d[1] + fr.t1z * d[2];
+   float y = fr.nx * d[0] + fr.ny * d[1] + fr.nz * d[2];
+   float z = fr.t2x * d[0] + fr.t2y * d[1] + fr.t2z * d[2];
+   normalize3(x, y, z);
+   lin[0] = x; lin[1] = y; lin[2] = z;
+}
+
+__device__ __forceinline__ bool src_is_zero(const float src[3]) {        // spectrum.h:65-67
+   return src[0] < 0.0039f && src[1] < 0.0039f && src[2] < 0.0039f;
+}
+
+__device__ __forceinline__ Frame frame_of(const DevScene &S, uint32_t tri, const float d[3], float u, float v,
+                                          float w, uint32_t after, uint32_t mesh) {
+   Ray r;
+   r.dx = d[0]; r.dy = d[1]; r.dz = d[2];
+   const float *cn = nullptr;
+   if (S.normals && S.meshes[mesh].has_normals) cn = S.normals + 9 * (size_t)tri;
+   Frame fr;
+   hit_frame(S.tris + 9 * (size_t)tri, cn, r, u, v, w, after, fr);
+   return fr;
+}
+
+// ---------------------------------------------------------------- primary hits
+
+// Record per pixel (16 words): w0 = kind | mesh << 2 (kind 0 = the path
+// bounces, 1 = it ends here with radiance w1..w3 for src = 1); for kind 0:
+// w1..3 Location, w4..6 tangent1, w7..9 normal, w10..12 tangent2, w13..15 the
+// normalised local incoming direction.
+struct PrimaryPolicy {
+   struct Ctx {
+      uint32_t dbg;
+   };
+   const JobDev *job;          // device pointer (per launch: one job)
+   DevScene S;
+   JobDev J;
+   __device__ __forceinline__ void load(uint32_t, Fetched &) const {}
+   __device__ __forceinline__ int start(uint32_t id, const Fetched &, Ctx &ctx, Trav &s) const {
+      const uint32_t x = id % J.width, y = id / J.width;
+      float o[3], d[3];
+      camera_ray(J, x, y, o, d);
+      make_ray(s.r, o[0], o[1], o[2], d[0], d[1], d[2]);
+      ctx.dbg = (x == 128 && y == 128) ? 1u : 0u;
+      return 0;
+   }
+   __device__ __forceinline__ void load2(Ctx &, Fetched &) const {}
+   __device__ __forceinline__ void start2(Ctx &, const Fetched &, Trav &) const {}
+   __device__ __forceinline__ void begin_root(Ctx &, Trav &s) const { begin(s, S.root_box, S.root_ref, S.root_count); }
+   __device__ __forceinline__ const Node64 *nodes(const Ctx &) const { return S.nodes; }
+   __device__ __forceinline__ const float *tris(const Ctx &) const { return S.tris; }
+   __device__ __forceinline__ bool debug_pixel(const Ctx &ctx) const { return ctx.dbg != 0; }
+   __device__ __forceinline__ void finish(uint32_t id, const Ctx &, const Trav &s) const {
+      float rec[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) rec[k] = 0.f;
+      const float one[3] = {1.f, 1.f, 1.f};
+      if (s.hit == kNone) {
+         const Radiance L = miss_radiance(S, one);
+         rec[0] = __uint_as_float(1u);
+         rec[1] = L.r; rec[2] = L.g; rec[3] = L.b;
+      } else {
+         const uint32_t mesh = S.pairs[2 * (size_t)s.hit];
+         const MeshRec &M = S.meshes[mesh];
+         if (M.is_light) {
+            rec[0] = __uint_as_float(1u | (mesh << 2));
+            rec[1] = (M.emission[0] + 0.f) * 1.f;
+            rec[2] = (M.emission[1] + 0.f) * 1.f;
+            rec[3] = (M.emission[2] + 0.f) * 1.f;
+         } else if (J.max_depth == 0) {
+            rec[0] = __uint_as_float(1u | (mesh << 2));
+         } else {
+            const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
+            const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
+            float lin[3];
+            world_to_local(fr, d, lin);
+            rec[0] = __uint_as_float(mesh << 2);
+            rec[1] = fr.lx; rec[2] = fr.ly; rec[3] = fr.lz;
+            rec[4] = fr.t1x; rec[5] = fr.t1y; rec[6] = fr.t1z;
+            rec[7] = fr.nx; rec[8] = fr.ny; rec[9] = fr.nz;
+            rec[10] = fr.t2x; rec[11] = fr.t2y; rec[12] = fr.t2z;
+            rec[13] = lin[0]; rec[14] = lin[1]; rec[15] = lin[2];
+         }
+      }
+      float4 *dst = reinterpret_cast<float4 *>(J.primary + 16 * (size_t)id);
+      dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
+      dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+      dst[2] = make_float4(rec[8], rec[9], rec[10], rec[11]);
+      dst[3] = make_float4(rec[12], rec[13], rec[14], rec[15]);
+   }
+};
+
+template <int kLds>
+__global__ void __launch_bounds__(256) k_primary(DevScene S, JobDev J, uint32_t *work, uint4 *spill) {
+   const PrimaryPolicy P{nullptr, S, J};
+   persistent_trace<kLds>(P, J.width * J.height, work, spill, 1u, S.max_stack);
+}
+
+// ---------------------------------------------------------------- path queues
+
+// Queue entry, 16 words: path slot, job, pixel, sample, origin xyz, direction
+// xyz, throughput rgb, flags (bit 0: the reference's debug pixel (128, 128)),
+// scene, unused.
+constexpr int kQWords = 16;
+
+struct QueueRefs {
+   float *q;                   // [capacity][16]
+   uint32_t *count;            // appended entries
+};
+
+// Appends one entry per calling lane with `want`; one atomic per wave.
+__device__ __forceinline__ void enqueue(const QueueRefs &Q, bool want, const float e[kQWords]) {
+   const uint64_t m = __ballot(want);
+   if (!m) return;
+   const uint32_t lane = __lane_id();
+   const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+   uint32_t base = 0;
+   if (lane == leader) base = atomicAdd(Q.count, (uint32_t)__popcll(m));
+   base = __shfl(base, leader);
+   if (want) {
+      const uint32_t slot = base + __popcll(m & ((1ull << lane) - 1ull));
+      float4 *dst = reinterpret_cast<float4 *>(Q.q + (size_t)kQWords * slot);
+      dst[0] = make_float4(e[0], e[1], e[2], e[3]);
+      dst[1] = make_float4(e[4], e[5], e[6], e[7]);
+      dst[2] = make_float4(e[8], e[9], e[10], e[11]);
+      dst[3] = make_float4(e[12], e[13], e[14], e[15]);
+   }
+}
+
+__device__ __forceinline__ void write_res(float *res, uint32_t path, Radiance L) {
+   res[3 * (size_t)path] = L.r;
+   res[3 * (size_t)path + 1] = L.g;
+   res[3 * (size_t)path + 2] = L.b;
+}
+
+__device__ __forceinline__ uint32_t find_chunk(const Chunk *chunks, uint32_t nchunks, uint32_t slot) {
+   uint32_t lo = 0, hi = nchunks;
+   while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (chunks[mid].base <= slot) lo = mid; else hi = mid;
+   }
+   return lo;
+}
+
+// Shades every path slot of a round at its primary hit (bounce 0).
+__global__ void __launch_bounds__(256) k_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots,
+                                                     const JobDev *jobs, const DevScene *scenes, float *res,
+                                                     QueueRefs out) {
+   for (uint32_t slot0 = blockIdx.x * blockDim.x; slot0 < nslots; slot0 += gridDim.x * blockDim.x) {
+      const uint32_t slot = slot0 + threadIdx.x;
+      bool want = false;
+      float e[kQWords];
+      if (slot < nslots) {
+         const Chunk c = chunks[find_chunk(chunks, nchunks, slot)];
+         const uint32_t local = slot - c.base;
+         const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
+         const JobDev &J = jobs[c.job];
+         const float4 *pr = reinterpret_cast<const float4 *>(J.primary + 16 * (size_t)pixel);
+         const float4 a = pr[0], b = pr[1], cc = pr[2], d = pr[3];
+         const uint32_t w0 = __float_as_uint(a.x);
+         if (w0 & 1u) {
+            write_res(res, slot, Radiance{a.y, a.z, a.w});
+         } else {
+            const DevScene &S = scenes[J.scene];
+            const MeshRec &M = S.meshes[w0 >> 2];
+            Frame fr;
+            fr.lx = a.y; fr.ly = a.z; fr.lz = a.w;
+            fr.t1x = b.x; fr.t1y = b.y; fr.t1z = b.z;
+            fr.nx = b.w; fr.ny = cc.x; fr.nz = cc.y;
+            fr.t2x = cc.z; fr.t2y = cc.w; fr.t2z = d.x;
+            const float lin[3] = {d.y, d.z, d.w};
+            float src[3] = {1.f, 1.f, 1.f};
+            float wout[3];
+            bounce(M, fr, lin, pixel, sample, 0u, J.seed, wout, src);
+            if (src_is_zero(src)) {
+               write_res(res, slot, Radiance{0.f, 0.f, 0.f});
+            } else {
+               want = true;
+               const uint32_t x = pixel % J.width, y = pixel / J.width;
+               e[0] = __uint_as_float(slot); e[1] = __uint_as_float(c.job);
+               e[2] = __uint_as_float(pixel); e[3] = __uint_as_float(sample);
+               e[4] = fr.lx; e[5] = fr.ly; e[6] = fr.lz;
+               e[7] = wout[0]; e[8] = wout[1]; e[9] = wout[2];
+               e[10] = src[0]; e[11] = src[1]; e[12] = src[2];
+               e[13] = __uint_as_float((x == 128 && y == 128) ? 1u : 0u);
+               e[14] = __uint_as_float(J.scene); e[15] = 0.f;
+            }
+         }
+      }
+      enqueue(out, want, e);
+   }
+}
+
+// Extend stage: closest hit of every queued path (persistent traversal).  Path
+// hit record (8 words): leaf-ordered triangle (kNone = miss), t, u, v, w, face
+// tests since the accepting one.
+struct ExtendPolicy {
+   struct Ctx {
+      uint32_t path, dbg, scene;
+      const Node64 *nodes;
+      const float *tris;
+   };
+   const float *queue;
+   const DevScene *scenes;
+   float *hits;
+   __device__ __forceinline__ void load(uint32_t id, Fetched &f) const {
+      const float4 *src = reinterpret_cast<const float4 *>(queue + (size_t)kQWords * id);
+      const float4 a = src[0], b = src[1], c = src[2], d = src[3];
+      f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
+      f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
+      f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
+      f.q[12] = d.x; f.q[13] = d.y; f.q[14] = d.z; f.q[15] = d.w;
+   }
+   __device__ __forceinline__ int start(uint32_t, const Fetched &f, Ctx &ctx, Trav &s) const {
+      make_ray(s.r, f.q[4], f.q[5], f.q[6], f.q[7], f.q[8], f.q[9]);
+      ctx.path = __float_as_uint(f.q[0]);
+      ctx.dbg = __float_as_uint(f.q[13]) & 1u;
+      ctx.scene = __float_as_uint(f.q[14]);
+      return 1;
+   }
+   __device__ __forceinline__ void load2(Ctx &ctx, Fetched &f) const {
+      const float4 *src = reinterpret_cast<const float4 *>(scenes + ctx.scene);
+      const float4 a = src[0], b = src[1], c = src[2];
+      f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
+      f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
+      f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
+   }
+   __device__ __forceinline__ void start2(Ctx &ctx, const Fetched &f, Trav &s) const {
+      const uint64_t np = (uint64_t)__float_as_uint(f.q[0]) | ((uint64_t)__float_as_uint(f.q[1]) << 32);
+      const uint64_t tp = (uint64_t)__float_as_uint(f.q[2]) | ((uint64_t)__float_as_uint(f.q[3]) << 32);
+      ctx.nodes = reinterpret_cast<const Node64 *>(np);
+      ctx.tris = reinterpret_cast<const float *>(tp);
+      begin(s, &f.q[4], __float_as_uint(f.q[10]), __float_as_uint(f.q[11]));
+   }
+   __device__ __forceinline__ void begin_root(Ctx &, Trav &) const {}
+   __device__ __forceinline__ const Node64 *nodes(const Ctx &ctx) const { return ctx.nodes; }
+   __device__ __forceinline__ const float *tris(const Ctx &ctx) const { return ctx.tris; }
+   __device__ __forceinline__ bool debug_pixel(const Ctx &ctx) const { return ctx.dbg != 0; }
+   __device__ __forceinline__ void finish(uint32_t, const Ctx &ctx, const Trav &s) const {
+      float4 *dst = reinterpret_cast<float4 *>(hits + 8 * (size_t)ctx.path);
+      dst[0] = make_float4(__uint_as_float(s.hit), s.min_t, s.hu, s.hv);
+      dst[1] = make_float4(s.hw, __uint_as_float(s.after), 0.f, 0.f);
+   }
+};
+
+template <int kLds>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_extend(const float *queue, const uint32_t *count, const DevScene *scenes, float *hits, uint32_t *work, uint4 *spill,
+         uint32_t max_stack) {
+   const ExtendPolicy P{queue, scenes, hits};
+   const uint32_t n = __builtin_amdgcn_readfirstlane(*count);
+   persistent_trace<kLds>(P, n, work, spill, 1u, max_stack);
+}
+
+// Shade stage at bounce b >= 1 (PathTraceIntegrator.cpp:23-95).
+__global__ void __launch_bounds__(256) k_shade(const float *queue_in, const uint32_t *count_in, const float *hits,
+                                               const JobDev *jobs, const DevScene *scenes, float *res,
+                                               QueueRefs out, const uint32_t *bounce_ptr) {
+   const uint32_t n = *count_in;
+   const uint32_t b = *bounce_ptr;
+   for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+      const uint32_t i = i0 + threadIdx.x;
+      bool want = false;
+      float e[kQWords];
+      if (i < n) {
+         const float4 *qe = reinterpret_cast<const float4 *>(queue_in + (size_t)kQWords * i);
+         const float4 a = qe[0], bq = qe[1], c = qe[2], d = qe[3];
+         const uint32_t path = __float_as_uint(a.x), job = __float_as_uint(a.y);
+         const uint32_t pixel = __float_as_uint(a.z), sample = __float_as_uint(a.w);
+         const float dir[3] = {bq.w, c.x, c.y};
+         float src[3] = {c.z, c.w, d.x};
+         const uint32_t scene = __float_as_uint(d.z);
+         const float4 *hp = reinterpret_cast<const float4 *>(hits + 8 * (size_t)path);
+         const float4 h0 = hp[0], h1 = hp[1];
+         const uint32_t tri = __float_as_uint(h0.x);
+         const DevScene &S = scenes[scene];
+         const JobDev &J = jobs[job];
+         if (tri == kNone) {
+            write_res(res, path, miss_radiance(S, src));
+         } else {
+            const uint32_t mesh = S.pairs[2 * (size_t)tri];
+            const MeshRec &M = S.meshes[mesh];
+            if (M.is_light) {
+               write_res(res, path, Radiance{(M.emission[0] + 0.f) * src[0], (M.emission[1] + 0.f) * src[1],
+                                             (M.emission[2] + 0.f) * src[2]});
+            } else if (b >= J.max_depth) {
+               write_res(res, path, Radiance{0.f, 0.f, 0.f});
+            } else {
+               const Frame fr = frame_of(S, tri, dir, h0.z, h0.w, h1.x, __float_as_uint(h1.y), mesh);
+               float lin[3], wout[3];
+               world_to_local(fr, dir, lin);
+               bounce(M, fr, lin, pixel, sample, b, J.seed, wout, src);
+               if (src_is_zero(src)) {
+                  write_res(res, path, Radiance{0.f, 0.f, 0.f});
+               } else {
+                  want = true;
+                  e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+                  e[4] = fr.lx; e[5] = fr.ly; e[6] = fr.lz;
+                  e[7] = wout[0]; e[8] = wout[1]; e[9] = wout[2];
+                  e[10] = src[0]; e[11] = src[1]; e[12] = src[2];
+                  e[13] = d.y; e[14] = d.z; e[15] = 0.f;
+               }
+            }
+         }
+      }
+      enqueue(out, want, e);
+   }
+}
+
+// Adds each chunk's samples to its job's film, per pixel in sample order, each
+// sample clamped to 255 per channel (BoxFilter::AddSample/Output).
+__global__ void __launch_bounds__(256) k_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs,
+                                                    const float *res) {
+   for (uint32_t ci = 0; ci < nchunks; ci++) {
+      const Chunk c = chunks[ci];
+      const JobDev &J = jobs[c.job];
+      for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < c.npix; p += gridDim.x * blockDim.x) {
+         float *film = J.film + 3 * (size_t)p;
+         float ar = film[0], ag = film[1], ab = film[2];
+         for (uint32_t s = 0; s < c.k; s++) {
+            const float *r = res + 3 * ((size_t)c.base + (size_t)s * c.npix + p);
+            float sr = r[0], sg = r[1], sb = r[2];
+            if (sr > 255) sr = 255;
+            if (sg > 255) sg = 255;
+            if (sb > 255) sb = 255;
+            ar += sr; ag += sg; ab += sb;
+         }
+         film[0] = ar; film[1] = ag; film[2] = ab;
+      }
+   }
+}
+
+__global__ void k_finalize(float *film, uint32_t npix, uint32_t spp) {
+   const float div = 1.0f / (float)spp;
+   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+      film[3 * (size_t)p] *= div;
+      film[3 * (size_t)p + 1] *= div;
+      film[3 * (size_t)p + 2] *= div;
+   }
+}
+
+// Start of a bounce: clears the traversal work counters and the next queue's
+// count, and advances the device-resident bounce index.
+__global__ void k_bounce_prologue(uint32_t *work, uint32_t *count_out, uint32_t *bounce) {
+   const uint32_t i = threadIdx.x;
+   if (i < 8) work[i] = 0u;
+   if (i == 8) *count_out = 0u;
+   if (i == 9) *bounce += 1u;
+}
+
+__global__ void k_zero_words(uint32_t *a, uint32_t na, uint32_t *b, uint32_t nb) {
+   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < na) a[i] = 0u;
+   if (b && i < nb) b[i] = 0u;
+}
+
+__global__ void k_zero_floats(float *a, size_t n) {
+   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+      a[i] = 0.f;
+}
+
+// ---------------------------------------------------------------- plain path
+
+// One sample of every pixel: one thread per pixel path, all bounces inline.
+template <int kLds>
+__global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sample, uint4 *spill) {
+   __shared__ uint32_t lds_ref[kLds * 256];
+   __shared__ float lds_near[kLds * 256];
+   __shared__ uint16_t lds_cnt[kLds * 256];
+   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
+                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, S.max_stack};
+   const uint32_t npix = J.width * J.height;
+   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+      const uint32_t x = p % J.width, y = p / J.width;
+      const bool dbg = x == 128 && y == 128;
+      float o[3], d[3];
+      camera_ray(J, x, y, o, d);
+      float src[3] = {1.f, 1.f, 1.f};
+      Radiance L = {0.f, 0.f, 0.f};
+      for (uint32_t b = 0;; b++) {
+         if (src_is_zero(src)) break;
+         Trav s;
+         make_ray(s.r, o[0], o[1], o[2], d[0], d[1], d[2]);
+         begin(s, S.root_box, S.root_ref, S.root_count);
+         while (s.cur != kNone) {
+            Fetched f;
+            fetch(s, S.nodes, S.tris, f);
+            if (!advance<kLds>(s, f, st, dbg)) break;
+         }
+         if (s.hit == kNone) {
+            L = miss_radiance(S, src);
+            break;
+         }
+         const uint32_t mesh = S.pairs[2 * (size_t)s.hit];
+         const MeshRec &M = S.meshes[mesh];
+         if (M.is_light) {
+            L = Radiance{(M.emission[0] + 0.f) * src[0], (M.emission[1] + 0.f) * src[1],
+                         (M.emission[2] + 0.f) * src[2]};
+            break;
+         }
+         if (b >= J.max_depth) break;
+         const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
+         float lin[3], wout[3];
+         world_to_local(fr, d, lin);
+         bounce(M, fr, lin, p, sample, b, J.seed, wout, src);
+         o[0] = fr.lx; o[1] = fr.ly; o[2] = fr.lz;
+         d[0] = wout[0]; d[1] = wout[1]; d[2] = wout[2];
+      }
+      float *film = J.film + 3 * (size_t)p;
+      float sr = L.r, sg = L.g, sb = L.b;
+      if (sr > 255) sr = 255;
+      if (sg > 255) sg = 255;
+      if (sb > 255) sb = 255;
+      film[0] += sr;
+      film[1] += sg;
+      film[2] += sb;
+   }
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------- launchers
+
+namespace {
+constexpr uint32_t kRenderLds = 12;
+}
+
+uint32_t render_lds_depth() { return kRenderLds; }
+
+int render_blocks_per_cu(int which) {
+   int b = 0;
+   hipError_t e = hipSuccess;
+   if (which == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_extend<kRenderLds>, 256, 0);
+   else if (which == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_primary<kRenderLds>, 256, 0);
+   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds>, 256, 0);
+   return e == hipSuccess && b > 0 ? b : 1;
+}
+
+void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t grid,
+                    hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
+   hipLaunchKernelGGL(dev::k_primary<kRenderLds>, dim3(grid), dim3(256), 0, st, S, J, work, spill);
+}
+
+void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs,
+                        const DevScene *scenes, float *res, float *q_out, uint32_t *count_out, uint32_t *bounce,
+                        uint32_t grid, hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, count_out, 1u, bounce, 1u);
+   hipLaunchKernelGGL(dev::k_round_start, dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, scenes, res,
+                      dev::QueueRefs{q_out, count_out});
+}
+
+// One bounce: prologue, extend, shade.  Captured once per queue direction and
+// replayed for every bounce.
+void launch_bounce(const float *q_in, const uint32_t *count_in, float *q_out, uint32_t *count_out, float *hits,
+                   const JobDev *jobs, const DevScene *scenes, float *res, uint32_t *work, uint4 *spill,
+                   uint32_t max_stack, uint32_t *bounce, uint32_t ext_grid, uint32_t shade_grid, hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_bounce_prologue, dim3(1), dim3(64), 0, st, work, count_out, bounce);
+   hipLaunchKernelGGL(dev::k_extend<kRenderLds>, dim3(ext_grid), dim3(256), 0, st, q_in, count_in, scenes, hits, work,
+                      spill, max_stack);
+   hipLaunchKernelGGL(dev::k_shade, dim3(shade_grid), dim3(256), 0, st, q_in, count_in, hits, jobs, scenes, res,
+                      dev::QueueRefs{q_out, count_out}, bounce);
+}
+
+void launch_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs, const float *res, uint32_t grid,
+                       hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_accumulate, dim3(grid), dim3(256), 0, st, chunks, nchunks, jobs, res);
+}
+
+void launch_finalize(float *film, uint32_t npix, uint32_t spp, hipStream_t st) {
+   const uint32_t grid = std::min<uint32_t>(4096, (npix + 255) / 256);
+   hipLaunchKernelGGL(dev::k_finalize, dim3(std::max(1u, grid)), dim3(256), 0, st, film, npix, spp);
+}
+
+void launch_zero_floats(float *a, size_t n, hipStream_t st) {
+   const uint32_t grid = (uint32_t)std::min<size_t>(4096, (n + 255) / 256);
+   hipLaunchKernelGGL(dev::k_zero_floats, dim3(std::max(1u, grid)), dim3(256), 0, st, a, n);
+}
+
+void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_mega<kRenderLds>, dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+}
+
+}  // namespace ptx

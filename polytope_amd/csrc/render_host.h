@@ -1,0 +1,57 @@
+// Render engine: device-resident path queues and the batched wavefront driver.
+#pragma once
+
+#include <vector>
+
+#include "scene.h"
+
+namespace ptx {
+
+struct RenderJob {
+   ptx_job_desc desc{};
+   uint32_t scene_index = 0;
+   uint32_t npix = 0;
+};
+
+int validate_job(const ptx_job_desc *j);
+// Device bytes a job holds while it runs (film + primary-hit records).
+size_t job_device_bytes(const ptx_job_desc &j);
+
+std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jobs, uint32_t cap);
+
+struct RenderEngine {
+   static constexpr uint32_t kDefaultPaths = 1u << 22;
+   // layout of `words`: 8 traversal work counters, the two queue counts, the bounce index
+   static constexpr uint32_t kWork = 0, kCountA = 8, kCountB = 9, kBounce = 10;
+
+   ptx_device *dev = nullptr;
+   hipStream_t stream = nullptr;
+   uint32_t cap = 0;                        // path slots
+   float *qa = nullptr, *qb = nullptr;      // queues, 16 words per entry
+   float *hits = nullptr;                   // 8 words per path slot
+   float *res = nullptr;                    // 3 floats per path slot
+   uint32_t *words = nullptr;
+   uint4 *spill = nullptr;
+   size_t spill_entries = 0;
+   uint32_t stack_cap = 0;
+   JobDev *d_jobs = nullptr;
+   uint32_t jobs_cap = 0;
+   DevScene *d_scenes = nullptr;
+   uint32_t scenes_cap = 0;
+   Chunk *d_chunks = nullptr;
+   uint32_t chunks_cap = 0;
+   uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0;
+   hipGraphExec_t g_ab = nullptr, g_ba = nullptr;
+
+   ~RenderEngine();
+   void release();
+   static size_t bytes_for(uint32_t paths);
+   int ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uint32_t nscenes, uint32_t nchunks);
+   int run_batch(const std::vector<RenderJob> &jobs, const std::vector<const ptx_scene *> &scenes,
+                 const std::vector<float *> &films_host);
+   int run_plain(const ptx_scene *scene, const ptx_job_desc &desc, float *film_host);
+};
+
+RenderEngine *engine_of(ptx_device *dev);
+
+}  // namespace ptx

@@ -18,6 +18,9 @@
 
 struct ptx_device;
 struct ptx_scene;
+namespace ptx {
+struct RenderEngine;
+}
 
 namespace ptx {
 
@@ -50,7 +53,9 @@ struct ptx_device {
    std::string arch;
    hipStream_t stream = nullptr;     // library stream (default priority)
    ptx_tuning tuning{};
-   std::mutex mu;
+   std::mutex mu;                            // serialises use of the library stream
+   std::unique_ptr<ptx::RenderEngine> engine;  // path queues and graphs, created on first render
+   ~ptx_device();
 };
 
 struct ptx_scene {

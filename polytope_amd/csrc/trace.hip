@@ -1,194 +1,13 @@
-// Closest-hit kernels for gfx950.
-//
-// One persistent kernel, one lane per ray in flight.  Waves fetch rays in
-// batches from eight per-XCD segments of the ray list (blockIdx % 8 is the
-// XCD the dispatcher places a block on), so coherent rays stay in one L2, and
-// refill idle lanes whenever enough have finished (ballot + mbcnt compaction:
-// one atomic per wave per refill).  The deferred-child stack lives in LDS
-// (`kLds` entries per lane); deeper entries spill to a global array sized from
-// the tree depth.
+// Closest-hit ray queries for gfx950 (ptx_intersect_closest*): the
+// persistent traversal of traverse.h over an array of rays, one scene.
 #include <hip/hip_runtime.h>
 
-#include "trace_device.h"
+#include "traverse.h"
 
 #pragma clang fp contract(off)
 
 namespace ptx {
 namespace dev {
-
-// Deferred-child stack of one lane: entries [0, kLds) in LDS, deeper ones in a
-// global spill array.  The LDS part is three arrays (reference, slab near
-// distance, leaf face count) indexed [entry][thread] so a wave's accesses hit
-// consecutive banks; spill entries go through non-temporal global accesses
-// (and so never share an instruction with the LDS path).
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-template <int kLds>
-struct Stack {
-   uint32_t *ref;      // __shared__ [kLds * 256]
-   float *near;        // __shared__ [kLds * 256]
-   uint16_t *cnt;      // __shared__ [kLds * 256]
-   u32x4 *spill;       // [max_stack - kLds][stride]
-   uint32_t tid, gtid, stride, max_stack;
-   __device__ __forceinline__ void put(uint32_t k, uint32_t r, float nr, uint32_t c) const {
-      if (k < (uint32_t)kLds) {
-         ref[k * 256 + tid] = r;
-         near[k * 256 + tid] = nr;
-         cnt[k * 256 + tid] = (uint16_t)c;
-      } else if (k < max_stack) {
-         const u32x4 e = {r, __float_as_uint(nr), c, 0u};
-         __builtin_nontemporal_store(e, &spill[(size_t)(k - kLds) * stride + gtid]);
-      }
-   }
-   __device__ __forceinline__ void get(uint32_t k, uint32_t &r, float &nr, uint32_t &c) const {
-      if (k < (uint32_t)kLds) {
-         r = ref[k * 256 + tid];
-         nr = near[k * 256 + tid];
-         c = cnt[k * 256 + tid];
-      } else if (k < max_stack) {
-         const u32x4 e = __builtin_nontemporal_load(&spill[(size_t)(k - kLds) * stride + gtid]);
-         r = e.x; nr = __uint_as_float(e.y); c = e.z;
-      } else {                     // unreachable: the tree depth bounds the stack
-         r = 0xFFFFFFFFu; nr = kInf; c = 0;
-      }
-   }
-};
-
-// Pops deferred children until one passes its pop-time test; returns false
-// when the stack is empty (traversal done).
-template <int kLds>
-__device__ __forceinline__ bool pop_next(Trav &s, const Stack<kLds> &st) {
-   while (s.sp > 0) {
-      s.sp--;
-      uint32_t r, c;
-      float nr;
-      st.get(s.sp, r, nr, c);
-      if (nr < 0.f || nr <= s.min_t) {
-         s.bb_hits++;
-         s.cur = r;
-         s.cnt = c;
-         return true;
-      }
-      s.bb_misses++;
-   }
-   s.cur = 0xFFFFFFFFu;
-   return false;
-}
-
-__device__ __forceinline__ void begin(Trav &s, const DevScene &sc) {
-   s.min_t = kInf;
-   s.hit = 0xFFFFFFFFu;
-   s.after = 0;
-   s.sp = 0;
-   s.bb_hits = s.bb_misses = s.tris = 0;
-   s.hu = s.hv = s.hw = 0.f;
-   bool inside;
-   float near, far;
-   slab(sc.root_box[0], sc.root_box[1], sc.root_box[2], sc.root_box[3], sc.root_box[4], sc.root_box[5],
-        s.r, inside, near, far);
-   if (inside || (near <= far && near <= s.min_t)) {
-      s.bb_hits++;
-      s.cur = sc.root_ref;
-      s.cnt = sc.root_count;
-   } else {
-      s.bb_misses++;
-      s.cur = 0xFFFFFFFFu;
-      s.cnt = 0;
-   }
-}
-
-// Memory operands of one traversal step, in one block of registers whatever
-// their kind: an interior record (q[0..11] child boxes, q[12..15] refs, counts,
-// axis), up to two leaf faces (q[0..8], q[9..17]) or a new ray (q[0..5]).
-struct Fetched {
-   float q[18];
-};
-
-__device__ __forceinline__ void fetch(const Trav &s, const DevScene &sc, Fetched &f) {
-   if (s.cur & kLeafBit) {
-      const float *p = sc.tris + 9 * (size_t)(s.cur & ~kLeafBit);
-#pragma unroll
-      for (int k = 0; k < 9; k++) f.q[k] = p[k];
-      if (s.cnt > 1) {
-#pragma unroll
-         for (int k = 0; k < 9; k++) f.q[9 + k] = p[9 + k];
-      }
-   } else {
-      const float4 *rec = reinterpret_cast<const float4 *>(sc.nodes + s.cur);
-      const float4 a = rec[0], b = rec[1], c = rec[2], m = rec[3];
-      f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
-      f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
-      f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
-      f.q[12] = m.x; f.q[13] = m.y; f.q[14] = m.z; f.q[15] = m.w;
-   }
-}
-
-// One step on fetched operands: test up to two faces of the leaf s.cur, or the
-// two children of the interior node s.cur.  Returns false when the traversal
-// has finished.
-template <int kLds>
-__device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<kLds> &st, bool debug_pixel) {
-   if (s.cur & kLeafBit) {
-      const uint32_t first = s.cur & ~kLeafBit;
-      const bool two = s.cnt > 1;
-      if (test_triangle(f.q, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
-         s.hit = first;
-         s.after = 0;
-      }
-      s.after++;
-      if (two) {
-         if (test_triangle(f.q + 9, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
-            s.hit = first + 1;
-            s.after = 0;
-         }
-         s.after++;
-      }
-      const uint32_t done = two ? 2u : 1u;
-      s.tris += done;
-      if (s.cnt > done) {            // more faces in this leaf: continue it next step
-         s.cur += done;
-         s.cnt -= done;
-         return true;
-      }
-      return pop_next(s, st);
-   }
-   bool in0, in1;
-   float n0, f0, n1, f1;
-   slab(f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5], s.r, in0, n0, f0);
-   slab(f.q[6], f.q[7], f.q[8], f.q[9], f.q[10], f.q[11], s.r, in1, n1, f1);
-   const uint32_t mref0 = __float_as_uint(f.q[12]), mref1 = __float_as_uint(f.q[13]);
-   const uint32_t mcounts = __float_as_uint(f.q[14]), maxis = __float_as_uint(f.q[15]);
-   const bool h0 = in0 || (n0 <= f0 && n0 <= s.min_t);
-   const bool h1 = in1 || (n1 <= f1 && n1 <= s.min_t);
-   // bvh.cpp:510-517: with neg_dir[axis] the high child (child 1) is popped first.
-   const bool high_first = (s.r.neg >> maxis) & 1u;
-   const bool fh = high_first ? h1 : h0, sh = high_first ? h0 : h1;
-   const uint32_t c0 = mcounts & 0xFFFFu, c1 = mcounts >> 16;
-   const uint32_t fref = high_first ? mref1 : mref0, sref = high_first ? mref0 : mref1;
-   const uint32_t fcnt = high_first ? c1 : c0, scnt = high_first ? c0 : c1;
-   const float snear = high_first ? (in0 ? -1.f : n0) : (in1 ? -1.f : n1);
-   if (fh) {
-      s.bb_hits++;
-      if (sh) {
-         st.put(s.sp, sref, snear, scnt);
-         s.sp++;
-      } else {
-         s.bb_misses++;
-      }
-      s.cur = fref;
-      s.cnt = fcnt;
-      return true;
-   }
-   s.bb_misses++;
-   if (sh) {
-      s.bb_hits++;
-      s.cur = sref;
-      s.cnt = scnt;
-      return true;
-   }
-   s.bb_misses++;
-   return pop_next(s, st);
-}
 
 __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, uint32_t id, ptx_hit *hits,
                                              ptx_ray_counters *counters, ptx_hit_frame *frames) {
@@ -219,103 +38,41 @@ __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, 
    }
 }
 
-constexpr int kSegments = 8;
-constexpr uint32_t kBatch = 64;     // ray ids taken per atomic
-
-// Per-wave source of ray ids: eight segments of the ray list, one per XCD
-// (blockIdx % 8 is the XCD a block runs on), each drained by an atomic counter
-// in batches of 64.  Lane 0 requests the next batch while the current one is
-// handed out, so the atomic's latency overlaps traversal steps.
-struct RaySource {
-   uint32_t n, seg, tries, pool_base, pool_left, prefetch;   // wave-uniform except prefetch (lane 0)
-   bool done;
-   __device__ __forceinline__ uint32_t seg_lo(uint32_t sg) const { return (uint32_t)((uint64_t)n * sg / kSegments); }
-   __device__ __forceinline__ uint32_t seg_size(uint32_t sg) const { return seg_lo(sg + 1) - seg_lo(sg); }
-   __device__ __forceinline__ void request(uint32_t *work, uint32_t lane) {
-      if (lane == 0) prefetch = atomicAdd(&work[seg], kBatch);
+struct RayQuery {
+   struct Ctx {};
+   DevScene sc;
+   const float *rays;
+   ptx_hit *hits;
+   ptx_ray_counters *counters;
+   ptx_hit_frame *frames;
+   __device__ __forceinline__ void load(uint32_t id, Fetched &f) const {
+      const float *rp = rays + 6 * (size_t)id;
+#pragma unroll
+      for (int k = 0; k < 6; k++) f.q[k] = rp[k];
    }
-   // Makes pool_left > 0 unless every segment is drained (then done = true).
-   __device__ __forceinline__ void refill(uint32_t *work, uint32_t lane) {
-      while (pool_left == 0 && !done) {
-         const uint32_t b = __builtin_amdgcn_readlane(prefetch, 0);
-         const uint32_t size = seg_size(seg);
-         if (b < size) {
-            pool_base = seg_lo(seg) + b;
-            pool_left = min(kBatch, size - b);
-            request(work, lane);
-         } else if (++tries < kSegments) {
-            seg = (seg + 1) % kSegments;
-            request(work, lane);
-         } else {
-            done = true;
-         }
-      }
+   __device__ __forceinline__ int start(uint32_t, const Fetched &f, Ctx &, Trav &s) const {
+      make_ray(s.r, f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5]);
+      return 0;
+   }
+   __device__ __forceinline__ void load2(Ctx &, Fetched &) const {}
+   __device__ __forceinline__ void start2(Ctx &, const Fetched &, Trav &) const {}
+   __device__ __forceinline__ void begin_root(Ctx &, Trav &s) const {
+      begin(s, sc.root_box, sc.root_ref, sc.root_count);
+   }
+   __device__ __forceinline__ const Node64 *nodes(const Ctx &) const { return sc.nodes; }
+   __device__ __forceinline__ const float *tris(const Ctx &) const { return sc.tris; }
+   __device__ __forceinline__ bool debug_pixel(const Ctx &) const { return false; }
+   __device__ __forceinline__ void finish(uint32_t id, const Ctx &, const Trav &s) const {
+      write_result(s, sc, id, hits, counters, frames);
    }
 };
 
-enum : uint32_t { kIdle = 0, kPending = 1, kActive = 2 };
-
 template <int kLds>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n,
-                                                     ptx_hit *__restrict__ hits, ptx_ray_counters *counters,
-                                                     ptx_hit_frame *frames, uint32_t *work, uint4 *spill,
-                                                     uint32_t refill_min) {
-   __shared__ uint32_t lds_ref[kLds * 256];
-   __shared__ float lds_near[kLds * 256];
-   __shared__ uint16_t lds_cnt[kLds * 256];
-   const uint32_t lane = __lane_id();
-   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
-                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, sc.max_stack};
-   RaySource src{n, blockIdx.x % kSegments, 0u, 0u, 0u, 0u, false};
-   src.request(work, lane);
-   Trav s;
-   uint32_t mode = kIdle;
-   uint32_t id = 0;
-   while (true) {
-      // hand out ray ids to idle lanes (no memory access unless a batch is used up)
-      uint64_t idle = __ballot(mode == kIdle);
-      if (idle && !src.done && (uint32_t)__popcll(idle) >= refill_min) {
-         while (idle) {
-            src.refill(work, lane);
-            if (src.done) break;
-            const uint32_t take = min((uint32_t)__popcll(idle), src.pool_left);
-            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
-            const bool mine = ((idle >> lane) & 1ull) && rank < take;
-            if (mine) {
-               id = src.pool_base + rank;
-               mode = kPending;
-            }
-            idle &= ~__ballot(mine);
-            src.pool_base += take;
-            src.pool_left -= take;
-         }
-      }
-      if (!__any(mode != kIdle)) break;
-      // fetch: the rays of newly assigned lanes and every active lane's operands
-      Fetched f;
-      if (mode == kPending) {
-         const float *rp = rays + 6 * (size_t)id;
-#pragma unroll
-         for (int k = 0; k < 6; k++) f.q[k] = rp[k];
-      } else if (mode == kActive) {
-         fetch(s, sc, f);
-      }
-      // compute
-      if (mode == kPending) {
-         make_ray(s.r, f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5]);
-         begin(s, sc);
-         mode = kActive;
-         if (s.cur == 0xFFFFFFFFu) {
-            write_result(s, sc, id, hits, counters, frames);
-            mode = kIdle;
-         }
-      } else if (mode == kActive) {
-         if (!advance<kLds>(s, f, st, false)) {
-            write_result(s, sc, id, hits, counters, frames);
-            mode = kIdle;
-         }
-      }
-   }
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n, ptx_hit *__restrict__ hits,
+              ptx_ray_counters *counters, ptx_hit_frame *frames, uint32_t *work, uint4 *spill, uint32_t refill_min) {
+   const RayQuery q{sc, rays, hits, counters, frames};
+   persistent_trace<kLds>(q, n, work, spill, refill_min, sc.max_stack);
 }
 
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {

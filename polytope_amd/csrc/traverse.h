@@ -1,0 +1,308 @@
+// Persistent closest-hit traversal shared by the ray-query kernel and the
+// render path's extend stage.
+//
+// One lane per ray in flight.  Each loop iteration is one step for every lane
+// of the wave: lanes that finished take new work ids from a per-wave pool,
+// then every lane issues the loads of its step (a new ray, an interior record
+// or up to two leaf faces) before any lane computes, so the wave waits on
+// memory once per step.  The deferred-child stack lives in LDS (kLds entries
+// per lane), deeper entries spill to a global array sized from the tree depth.
+#pragma once
+
+#include "trace_device.h"
+
+#pragma clang fp contract(off)
+
+namespace ptx {
+namespace dev {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int kLds>
+struct Stack {
+   uint32_t *ref;      // __shared__ [kLds * 256]
+   float *near;        // __shared__ [kLds * 256]
+   uint16_t *cnt;      // __shared__ [kLds * 256]
+   u32x4 *spill;       // [max_stack - kLds][stride]
+   uint32_t tid, gtid, stride, max_stack;
+   __device__ __forceinline__ void put(uint32_t k, uint32_t r, float nr, uint32_t c) const {
+      if (k < (uint32_t)kLds) {
+         ref[k * 256 + tid] = r;
+         near[k * 256 + tid] = nr;
+         cnt[k * 256 + tid] = (uint16_t)c;
+      } else if (k < max_stack) {
+         const u32x4 e = {r, __float_as_uint(nr), c, 0u};
+         spill[(size_t)(k - kLds) * stride + gtid] = e;
+      }
+   }
+   __device__ __forceinline__ void get(uint32_t k, uint32_t &r, float &nr, uint32_t &c) const {
+      if (k < (uint32_t)kLds) {
+         r = ref[k * 256 + tid];
+         nr = near[k * 256 + tid];
+         c = cnt[k * 256 + tid];
+      } else if (k < max_stack) {
+         const u32x4 e = spill[(size_t)(k - kLds) * stride + gtid];
+         r = e.x; nr = __uint_as_float(e.y); c = e.z;
+      } else {                     // unreachable: the tree depth bounds the stack
+         r = 0xFFFFFFFFu; nr = kInf; c = 0;
+      }
+   }
+};
+
+// Pops deferred children until one passes its pop-time test; returns false
+// when the stack is empty (traversal done).
+template <int kLds>
+__device__ __forceinline__ bool pop_next(Trav &s, const Stack<kLds> &st) {
+   while (s.sp > 0) {
+      s.sp--;
+      uint32_t r, c;
+      float nr;
+      st.get(s.sp, r, nr, c);
+      if (nr < 0.f || nr <= s.min_t) {
+         s.bb_hits++;
+         s.cur = r;
+         s.cnt = c;
+         return true;
+      }
+      s.bb_misses++;
+   }
+   s.cur = 0xFFFFFFFFu;
+   return false;
+}
+
+__device__ __forceinline__ void begin(Trav &s, const float *root_box, uint32_t root_ref, uint32_t root_count) {
+   s.min_t = kInf;
+   s.hit = 0xFFFFFFFFu;
+   s.after = 0;
+   s.sp = 0;
+   s.bb_hits = s.bb_misses = s.tris = 0;
+   s.hu = s.hv = s.hw = 0.f;
+   bool inside;
+   float near, far;
+   slab(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], s.r, inside, near, far);
+   if (inside || (near <= far && near <= s.min_t)) {
+      s.bb_hits++;
+      s.cur = root_ref;
+      s.cnt = root_count;
+   } else {
+      s.bb_misses++;
+      s.cur = 0xFFFFFFFFu;
+      s.cnt = 0;
+   }
+}
+
+// Memory operands of one traversal step, in one block of registers whatever
+// their kind: an interior record (q[0..11] child boxes, q[12..15] refs, counts,
+// axis), up to two leaf faces (q[0..8], q[9..17]) or a new ray (q[0..5]).
+struct Fetched {
+   float q[18];
+};
+
+__device__ __forceinline__ void fetch(const Trav &s, const Node64 *nodes, const float *tris, Fetched &f) {
+   if (s.cur & kLeafBit) {
+      const float *p = tris + 9 * (size_t)(s.cur & ~kLeafBit);
+#pragma unroll
+      for (int k = 0; k < 9; k++) f.q[k] = p[k];
+      if (s.cnt > 1) {
+#pragma unroll
+         for (int k = 0; k < 9; k++) f.q[9 + k] = p[9 + k];
+      }
+   } else {
+      const float4 *rec = reinterpret_cast<const float4 *>(nodes + s.cur);
+      const float4 a = rec[0], b = rec[1], c = rec[2], m = rec[3];
+      f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
+      f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
+      f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
+      f.q[12] = m.x; f.q[13] = m.y; f.q[14] = m.z; f.q[15] = m.w;
+   }
+}
+
+// One step on fetched operands: test up to two faces of the leaf s.cur, or the
+// two children of the interior node s.cur.  Returns false when the traversal
+// has finished.
+template <int kLds>
+__device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<kLds> &st, bool debug_pixel) {
+   if (s.cur & kLeafBit) {
+      const uint32_t first = s.cur & ~kLeafBit;
+      const bool two = s.cnt > 1;
+      if (test_triangle(f.q, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
+         s.hit = first;
+         s.after = 0;
+      }
+      s.after++;
+      if (two) {
+         if (test_triangle(f.q + 9, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
+            s.hit = first + 1;
+            s.after = 0;
+         }
+         s.after++;
+      }
+      const uint32_t done = two ? 2u : 1u;
+      s.tris += done;
+      if (s.cnt > done) {            // more faces in this leaf: continue it next step
+         s.cur += done;
+         s.cnt -= done;
+         return true;
+      }
+      return pop_next(s, st);
+   }
+   bool in0, in1;
+   float n0, f0, n1, f1;
+   slab(f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5], s.r, in0, n0, f0);
+   slab(f.q[6], f.q[7], f.q[8], f.q[9], f.q[10], f.q[11], s.r, in1, n1, f1);
+   const uint32_t mref0 = __float_as_uint(f.q[12]), mref1 = __float_as_uint(f.q[13]);
+   const uint32_t mcounts = __float_as_uint(f.q[14]), maxis = __float_as_uint(f.q[15]);
+   const bool h0 = in0 || (n0 <= f0 && n0 <= s.min_t);
+   const bool h1 = in1 || (n1 <= f1 && n1 <= s.min_t);
+   // bvh.cpp:510-517: with neg_dir[axis] the high child (child 1) is popped first.
+   const bool high_first = (s.r.neg >> maxis) & 1u;
+   const bool fh = high_first ? h1 : h0, sh = high_first ? h0 : h1;
+   const uint32_t c0 = mcounts & 0xFFFFu, c1 = mcounts >> 16;
+   const uint32_t fref = high_first ? mref1 : mref0, sref = high_first ? mref0 : mref1;
+   const uint32_t fcnt = high_first ? c1 : c0, scnt = high_first ? c0 : c1;
+   const float snear = high_first ? (in0 ? -1.f : n0) : (in1 ? -1.f : n1);
+   if (fh) {
+      s.bb_hits++;
+      if (sh) {
+         st.put(s.sp, sref, snear, scnt);
+         s.sp++;
+      } else {
+         s.bb_misses++;
+      }
+      s.cur = fref;
+      s.cnt = fcnt;
+      return true;
+   }
+   s.bb_misses++;
+   if (sh) {
+      s.bb_hits++;
+      s.cur = sref;
+      s.cnt = scnt;
+      return true;
+   }
+   s.bb_misses++;
+   return pop_next(s, st);
+}
+
+constexpr int kSegments = 8;
+constexpr uint32_t kBatch = 64;     // ray ids taken per atomic
+
+// Per-wave source of ray ids: eight segments of the ray list, one per XCD
+// (blockIdx % 8 is the XCD a block runs on), each drained by an atomic counter
+// in batches of 64.  Lane 0 requests the next batch while the current one is
+// handed out, so the atomic's latency overlaps traversal steps.
+struct RaySource {
+   uint32_t n, seg, tries, pool_base, pool_left, prefetch;   // wave-uniform except prefetch (lane 0)
+   bool done;
+   __device__ __forceinline__ uint32_t seg_lo(uint32_t sg) const { return (uint32_t)((uint64_t)n * sg / kSegments); }
+   __device__ __forceinline__ uint32_t seg_size(uint32_t sg) const { return seg_lo(sg + 1) - seg_lo(sg); }
+   __device__ __forceinline__ void request(uint32_t *work, uint32_t lane) {
+      if (lane == 0) prefetch = atomicAdd(&work[seg], kBatch);
+   }
+   // Makes pool_left > 0 unless every segment is drained (then done = true).
+   __device__ __forceinline__ void refill(uint32_t *work, uint32_t lane) {
+      while (pool_left == 0 && !done) {
+         const uint32_t b = __builtin_amdgcn_readlane(prefetch, 0);
+         const uint32_t size = seg_size(seg);
+         if (b < size) {
+            pool_base = seg_lo(seg) + b;
+            pool_left = min(kBatch, size - b);
+            request(work, lane);
+         } else if (++tries < kSegments) {
+            seg = (seg + 1) % kSegments;
+            request(work, lane);
+         } else {
+            done = true;
+         }
+      }
+   }
+};
+
+enum : uint32_t { kIdle = 0, kPending = 1, kStarting = 2, kActive = 3 };
+
+// Runs the persistent loop.  Policy supplies:
+//   Ctx                                   per-lane work context
+//   load(id, f)                           issue the loads of work item id into f.q
+//   start(id, f, ctx, s) -> int           set s.r and ctx from the loaded item; return
+//                                         0 = ready (uniform scene roots), 1 = needs a
+//                                         second load (load2), 2 = done without tracing
+//   load2(ctx, f) / start2(ctx, f, s)     optional second load (per-lane scene roots)
+//   begin_root(ctx, s)                    root box test for uniform-scene policies
+//   nodes(ctx), tris(ctx)                 the lane's scene arrays
+//   debug_pixel(ctx)                      reference debug-pixel quirk flag
+//   finish(id, ctx, s)                    write the result
+template <int kLds, class Policy>
+__device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, uint32_t *work, uint4 *spill,
+                                                 uint32_t refill_min, uint32_t max_stack) {
+   __shared__ uint32_t lds_ref[kLds * 256];
+   __shared__ float lds_near[kLds * 256];
+   __shared__ uint16_t lds_cnt[kLds * 256];
+   const uint32_t lane = __lane_id();
+   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
+                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, max_stack};
+   RaySource src{n, blockIdx.x % kSegments, 0u, 0u, 0u, 0u, n == 0};
+   if (!src.done) src.request(work, lane);
+   Trav s;
+   typename Policy::Ctx ctx;
+   uint32_t mode = kIdle;
+   uint32_t id = 0;
+   while (true) {
+      uint64_t idle = __ballot(mode == kIdle);
+      if (idle && !src.done && (uint32_t)__popcll(idle) >= refill_min) {
+         while (idle) {
+            src.refill(work, lane);
+            if (src.done) break;
+            const uint32_t take = min((uint32_t)__popcll(idle), src.pool_left);
+            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+            const bool mine = ((idle >> lane) & 1ull) && rank < take;
+            if (mine) {
+               id = src.pool_base + rank;
+               mode = kPending;
+            }
+            idle &= ~__ballot(mine);
+            src.pool_base += take;
+            src.pool_left -= take;
+         }
+      }
+      if (!__any(mode != kIdle)) break;
+      Fetched f;
+      if (mode == kPending) {
+         P.load(id, f);
+      } else if (mode == kStarting) {
+         P.load2(ctx, f);
+      } else if (mode == kActive) {
+         fetch(s, P.nodes(ctx), P.tris(ctx), f);
+      }
+      if (mode == kPending) {
+         const int r = P.start(id, f, ctx, s);
+         if (r == 2) {
+            P.finish(id, ctx, s);
+            mode = kIdle;
+         } else if (r == 1) {
+            mode = kStarting;
+         } else {
+            P.begin_root(ctx, s);
+            mode = kActive;
+            if (s.cur == 0xFFFFFFFFu) {
+               P.finish(id, ctx, s);
+               mode = kIdle;
+            }
+         }
+      } else if (mode == kStarting) {
+         P.start2(ctx, f, s);
+         mode = kActive;
+         if (s.cur == 0xFFFFFFFFu) {
+            P.finish(id, ctx, s);
+            mode = kIdle;
+         }
+      } else if (mode == kActive) {
+         if (!advance<kLds>(s, f, st, P.debug_pixel(ctx))) {
+            P.finish(id, ctx, s);
+            mode = kIdle;
+         }
+      }
+   }
+}
+
+}  // namespace dev
+}  // namespace ptx

@@ -1,0 +1,111 @@
+"""GPU parity of the render paths (ptx_render: plain, ptx_render_batch:
+batched wavefront) against the reference's golden film and the oracle.
+
+Bar: per-pixel relative error <= 1e-4 against the oracle under the same RNG
+stream (the paths share the oracle's operation order, so the films are checked
+bit for bit as well); the deterministic mirror scene matches the reference's
+own film bit for bit; every job renders the same alone and batched with others.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from polytope_amd import Device, Scene, render, render_batch
+from polytope_amd import scenes as S
+from tests import _fixtures as F
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+REL_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    d = Device(0)
+    yield d
+    d.close()
+
+
+def rel_err(a, b):
+    return float((np.abs(a - b) / np.maximum(np.abs(b), 1e-3)).max())
+
+
+def test_render_mirror_matches_reference_golden(dev):
+    d = F.load("mirror_render")
+    cam, spp, depth = F.camera_of(d)
+    print("case mirror golden", flush=True)
+    sc = Scene(dev, F.meshes_of(d), d["sky"])
+    plain = render(sc, cam, spp, depth, seed=1)
+    assert np.array_equal(plain.view(np.uint32), d["ref_film"].view(np.uint32))
+    (batched,) = render_batch([(sc, cam, spp, depth, 1)])
+    assert np.array_equal(batched.view(np.uint32), d["ref_film"].view(np.uint32))
+
+
+def _lambert_scene(w=80, h=60):
+    meshes, cam, sky = S.plain_render_scene(1, w, h)
+    return meshes, cam, sky
+
+
+@pytest.mark.parametrize("spp,depth", [(4, 5), (3, 1), (2, 0), (2, 8)])
+def test_render_plain_and_batched_match_oracle(dev, spp, depth):
+    meshes, cam, sky = _lambert_scene()
+    print(f"case lambert spp={spp} depth={depth}", flush=True)
+    ref = O.OracleScene(meshes, sky).render(cam, spp, depth, seed=5)
+    sc = Scene(dev, meshes, sky)
+    plain = render(sc, cam, spp, depth, seed=5)
+    (batched,) = render_batch([(sc, cam, spp, depth, 5)])
+    assert rel_err(plain, ref) <= REL_TOL
+    assert rel_err(batched, ref) <= REL_TOL
+    assert np.array_equal(plain.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
+
+
+def test_render_mixed_materials_emitter_and_debug_pixel(dev):
+    # mirror + Lambert + emitter, a film wider than 128 so pixel (128, 128) exists
+    v, f = S.icosphere(3, radius=1.0, center=(0.0, 1.0, 0.0), noise=0.05, seed=3)
+    gv, gf = S.quad(6.0, 0.0)
+    lv, lf = S.quad(1.5, 4.0)
+    meshes = [S.MeshData(v, f, brdf=S.MIRROR, refl=(0.9, 0.8, 0.7)),
+              S.MeshData(gv, gf, refl=(0.5, 0.6, 0.7)),
+              S.MeshData(lv, lf, is_light=True, emission=(255.0, 240.0, 200.0))]
+    cam = S.Camera(S.look_at((0.0, 2.0, -5.0), (0.0, 0.8, 0.0)), 60.0, 160, 136)
+    sky = (40.0, 60.0, 90.0)
+    ref = O.OracleScene(meshes, sky).render(cam, 4, 6, seed=9)
+    sc = Scene(dev, meshes, sky)
+    plain = render(sc, cam, 4, 6, seed=9)
+    (batched,) = render_batch([(sc, cam, 4, 6, 9)])
+    assert np.array_equal(plain.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
+
+
+def test_render_batch_alone_equals_mixed(dev):
+    # jobs of uneven resolution, spp and depth over two scenes: each film is the same
+    # rendered alone, in the mix, and with a path capacity small enough to force many rounds
+    meshes_a, cam_a, sky_a = _lambert_scene(64, 48)
+    v, f = S.icosphere(4, noise=0.05, seed=8)
+    meshes_b = [S.MeshData(v, f, refl=(0.7, 0.5, 0.3)),
+                S.MeshData(*S.quad(4.0, -1.1), brdf=S.MIRROR, refl=(0.8, 0.8, 0.8))]
+    sa = Scene(dev, meshes_a, sky_a)
+    sb = Scene(dev, meshes_b, (100.0, 120.0, 150.0))
+    cam_b = S.Camera(S.look_at((0.3, 0.5, -3.0), (0.0, 0.0, 0.0)), 50.0, 40, 30)
+    cam_c = S.Camera(S.look_at((0.3, 0.5, -3.0), (0.0, 0.0, 0.0)), 70.0, 96, 40)
+    jobs = [(sa, cam_a, 5, 4, 11), (sb, cam_b, 9, 7, 12), (sb, cam_c, 2, 3, 13), (sa, cam_a, 1, 2, 14)]
+    alone = [render_batch([j])[0] for j in jobs]
+    mixed = render_batch(jobs)
+    dev.set_tuning(paths_per_wave=5000)
+    try:
+        small = render_batch(jobs)
+    finally:
+        dev.set_tuning()
+    for a, m, s, j in zip(alone, mixed, small, jobs):
+        assert np.array_equal(a.view(np.uint32), m.view(np.uint32))
+        assert np.array_equal(a.view(np.uint32), s.view(np.uint32))
+        plain = render(*j)
+        assert np.array_equal(a.view(np.uint32), plain.view(np.uint32))
+    ref = O.OracleScene(meshes_b, (100.0, 120.0, 150.0)).render(cam_b, 9, 7, seed=12)
+    assert np.array_equal(mixed[1].view(np.uint32), ref.view(np.uint32))
