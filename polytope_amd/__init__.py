@@ -20,7 +20,7 @@ from . import _abi
 from ._abi import (BVH_NODE_DTYPE, COUNTER_DTYPE, FRAME_DTYPE, HIT_DTYPE, PtxError, check, load)
 from .scenes import Camera, MeshData
 
-__all__ = ["Device", "Scene", "PtxError", "render", "render_batch", "HIT_DTYPE", "COUNTER_DTYPE", "FRAME_DTYPE", "BVH_NODE_DTYPE",
+__all__ = ["Device", "Scene", "PtxError", "render", "render_batch", "Server", "autotune", "HIT_DTYPE", "COUNTER_DTYPE", "FRAME_DTYPE", "BVH_NODE_DTYPE",
            "mesh_descs", "build_bvh", "camera_desc", "job_desc", "MeshData", "Camera"]
 
 
@@ -218,3 +218,68 @@ def render_batch(jobs: Sequence[Tuple[Scene, Camera, int, int, int]]) -> List[np
     ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in films])
     check(load().ptx_render_batch(handles, descs, n, ptrs))
     return films
+
+
+def autotune(device: Device, scene: Scene) -> dict:
+    """Times traversal (LDS stack depth x refill threshold) and wavefront path
+    capacity on ``scene`` and keeps the fastest settings on ``device``."""
+    t = _abi.Tuning()
+    check(load().ptx_autotune(device.handle, scene.handle, C.byref(t)))
+    return {k: getattr(t, k) for k, _ in _abi.Tuning._fields_}
+
+
+def _status_dict(st: _abi.JobStatus) -> dict:
+    return {k: getattr(st, k) for k, _ in _abi.JobStatus._fields_}
+
+
+class Server:
+    """Continuous-batching render service (ptx_server_*): jobs are admitted as
+    HBM budget and job slots free up and share the GPU round by round."""
+
+    def __init__(self, device: Device, hbm_budget_bytes: int, max_jobs_in_flight: int = 0,
+                 paths_per_wave: int = 0):
+        cfg = _abi.ServerConfig(hbm_budget_bytes, max_jobs_in_flight, paths_per_wave)
+        h = C.c_void_p()
+        check(load().ptx_server_create(device.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self.device = device
+        self._dims = {}
+
+    def submit(self, scene: Scene, cam: Camera, spp: int, max_depth: int, seed: int) -> int:
+        job = job_desc(cam, spp, max_depth, seed)
+        jid = C.c_uint64()
+        check(load().ptx_server_submit(self.handle, scene.handle, C.byref(job), C.byref(jid)))
+        self._dims[jid.value] = (cam.height, cam.width)
+        return jid.value
+
+    def status(self, job_id: int) -> dict:
+        st = _abi.JobStatus()
+        check(load().ptx_server_status(self.handle, job_id, C.byref(st)))
+        return _status_dict(st)
+
+    def wait(self, job_id: int):
+        """Blocks until the job ends; returns (film, status).  Raises PtxError for
+        failed or cancelled jobs (status attached as .status)."""
+        h, w = self._dims.pop(job_id)
+        film = np.zeros((h, w, 3), np.float32)
+        st = _abi.JobStatus()
+        rc = load().ptx_server_wait(self.handle, job_id, film.ctypes.data, C.byref(st))
+        if rc != _abi.PTX_OK:
+            err = PtxError(rc, load().ptx_last_error().decode(errors="replace"))
+            err.status = _status_dict(st)
+            raise err
+        return film, _status_dict(st)
+
+    def cancel(self, job_id: int) -> None:
+        check(load().ptx_server_cancel(self.handle, job_id))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            load().ptx_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

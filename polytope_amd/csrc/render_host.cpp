@@ -39,6 +39,8 @@ void dfree(T *&p) {
    p = nullptr;
 }
 
+}  // namespace
+
 JobDev job_dev(const ptx_job_desc &j, uint32_t scene_index) {
    JobDev J{};
    std::memcpy(J.cam, j.camera.camera_to_world, sizeof(J.cam));
@@ -54,8 +56,6 @@ JobDev job_dev(const ptx_job_desc &j, uint32_t scene_index) {
    J.scene = scene_index;
    return J;
 }
-
-}  // namespace
 
 int validate_job(const ptx_job_desc *j) {
    if (!j) return set_error(PTX_ERR_INVALID, "NULL job");
@@ -196,6 +196,44 @@ std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jo
    return rounds;
 }
 
+int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
+   int rc;
+   if ((rc = dalloc(&J.film, (size_t)npix * 3)) || (rc = dalloc(&J.primary, (size_t)npix * 16))) {
+      dfree(J.film);
+      return rc;
+   }
+   launch_zero_floats(J.film, (size_t)npix * 3, stream);
+   launch_primary(S, J, words + kWork, spill, std::min<uint32_t>(ext_grid, (npix + 255) / 256), stream);
+   return PTX_OK;
+}
+
+void RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth) {
+   uint32_t nslots = 0;
+   for (const Chunk &c : r) nslots = std::max(nslots, c.base + c.k * c.npix);
+   launch_round_start(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, res, qa, words + kCountA,
+                      words + kBounce, std::min<uint32_t>(shade_grid, (nslots + 255) / 256), stream);
+   for (uint32_t b = 1; b <= depth; b++) (void)hipGraphLaunch(b & 1 ? g_ab : g_ba, stream);
+   launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
+}
+
+int RenderEngine::finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host) {
+   launch_finalize(J.film, npix, spp, stream);
+   if (film_host)
+      PTX_HIP_TRY(hipMemcpyAsync(film_host, J.film, (size_t)npix * 3 * sizeof(float), hipMemcpyDeviceToHost, stream));
+   return PTX_OK;
+}
+
+void RenderEngine::free_job(JobDev &J) {
+   dfree(J.film);
+   dfree(J.primary);
+}
+
+uint32_t RenderEngine::capacity_for(uint64_t total_paths, uint32_t max_npix) const {
+   uint32_t want = dev->tuning.paths_per_wave ? dev->tuning.paths_per_wave : kDefaultPaths;
+   want = (uint32_t)std::min<uint64_t>(want, std::max<uint64_t>(total_paths, max_npix));
+   return std::max(std::max(want, max_npix), cap);
+}
+
 int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vector<const ptx_scene *> &scenes,
                             const std::vector<float *> &films_host) {
    const uint32_t nj = (uint32_t)jobs.size();
@@ -206,35 +244,27 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
       max_npix = std::max(max_npix, j.npix);
       total_paths += (uint64_t)j.npix * j.desc.spp;
    }
-   uint32_t want = dev->tuning.paths_per_wave ? dev->tuning.paths_per_wave : kDefaultPaths;
-   want = (uint32_t)std::min<uint64_t>(want, std::max<uint64_t>(total_paths, max_npix));
-   want = std::max(want, max_npix);
-   const auto rounds = schedule_rounds(jobs, std::max(want, cap));
+   const uint32_t paths = capacity_for(total_paths, max_npix);
+   const auto rounds = schedule_rounds(jobs, paths);
    size_t nchunks = 0;
    for (const auto &r : rounds) nchunks += r.size();
-   int rc = ensure(std::max(want, cap), max_stack, nj, (uint32_t)scenes.size(), (uint32_t)nchunks);
+   int rc = ensure(paths, max_stack, nj, (uint32_t)scenes.size(), (uint32_t)nchunks);
    if (rc) return rc;
    hipStream_t st = stream;
-   // per-job device buffers
    std::vector<JobDev> jd(nj);
-   std::vector<float *> films(nj, nullptr);
-   std::vector<uint32_t *> prim(nj, nullptr);
    auto cleanup = [&]() {
-      for (auto &f : films) dfree(f);
-      for (auto &p : prim) dfree(p);
+      for (auto &J : jd) free_job(J);
    };
+   std::vector<DevScene> sd(scenes.size());
+   for (size_t i = 0; i < scenes.size(); i++) sd[i] = scenes[i]->d;
    for (uint32_t j = 0; j < nj; j++) {
-      if ((rc = dalloc(&films[j], (size_t)jobs[j].npix * 3)) || (rc = dalloc(&prim[j], (size_t)jobs[j].npix * 16))) {
+      jd[j] = job_dev(jobs[j].desc, jobs[j].scene_index);
+      if ((rc = start_job(jd[j], sd[jobs[j].scene_index], jobs[j].npix))) {
+         (void)hipStreamSynchronize(st);
          cleanup();
          return rc;
       }
-      jd[j] = job_dev(jobs[j].desc, jobs[j].scene_index);
-      jd[j].film = films[j];
-      jd[j].primary = prim[j];
-      launch_zero_floats(films[j], (size_t)jobs[j].npix * 3, st);
    }
-   std::vector<DevScene> sd(scenes.size());
-   for (size_t i = 0; i < scenes.size(); i++) sd[i] = scenes[i]->d;
    std::vector<Chunk> all;
    all.reserve(nchunks);
    for (const auto &r : rounds) all.insert(all.end(), r.begin(), r.end());
@@ -242,28 +272,19 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
    PTX_HIP_TRY(hipMemcpyAsync(d_scenes, sd.data(), sd.size() * sizeof(DevScene), hipMemcpyHostToDevice, st));
    if (!all.empty())
       PTX_HIP_TRY(hipMemcpyAsync(d_chunks, all.data(), all.size() * sizeof(Chunk), hipMemcpyHostToDevice, st));
-   // primary hits, one traversal per pixel and job
-   for (uint32_t j = 0; j < nj; j++)
-      launch_primary(sd[jobs[j].scene_index], jd[j], words + kWork, spill,
-                     std::min<uint32_t>(ext_grid, (jobs[j].npix + 255) / 256), st);
    size_t off = 0;
    for (const auto &r : rounds) {
-      uint32_t nslots = 0, depth = 0;
-      for (const Chunk &c : r) {
-         nslots = std::max(nslots, c.base + c.k * c.npix);
-         depth = std::max(depth, jobs[c.job].desc.max_depth);
-      }
-      launch_round_start(d_chunks + off, (uint32_t)r.size(), nslots, d_jobs, d_scenes, res, qa, words + kCountA,
-                         words + kBounce, std::min<uint32_t>(shade_grid, (nslots + 255) / 256), st);
-      for (uint32_t b = 1; b <= depth; b++) PTX_HIP_TRY(hipGraphLaunch(b & 1 ? g_ab : g_ba, st));
-      launch_accumulate(d_chunks + off, (uint32_t)r.size(), d_jobs, res, shade_grid, st);
+      uint32_t depth = 0;
+      for (const Chunk &c : r) depth = std::max(depth, jobs[c.job].desc.max_depth);
+      round(d_chunks + off, r, depth);
       off += r.size();
    }
    for (uint32_t j = 0; j < nj; j++) {
-      launch_finalize(films[j], jobs[j].npix, jobs[j].desc.spp, st);
-      if (films_host[j])
-         PTX_HIP_TRY(hipMemcpyAsync(films_host[j], films[j], (size_t)jobs[j].npix * 3 * sizeof(float),
-                                    hipMemcpyDeviceToHost, st));
+      if ((rc = finish_job(jd[j], jobs[j].npix, jobs[j].desc.spp, films_host[j]))) {
+         (void)hipStreamSynchronize(st);
+         cleanup();
+         return rc;
+      }
    }
    const hipError_t se = hipStreamSynchronize(st);
    cleanup();
@@ -344,3 +365,130 @@ int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- autotuner
+
+namespace ptx {
+namespace {
+
+// Camera-to-world looking at `target` from `eye` (left-handed, +z forward).
+void look_at(const float eye[3], const float target[3], float m[16]) {
+   float d[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
+   float l = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+   for (float &x : d) x /= l;
+   const float up[3] = {0.f, 1.f, 0.f};
+   float r[3] = {up[1] * d[2] - up[2] * d[1], up[2] * d[0] - up[0] * d[2], up[0] * d[1] - up[1] * d[0]};
+   l = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+   for (float &x : r) x /= l;
+   const float u[3] = {d[1] * r[2] - d[2] * r[1], d[2] * r[0] - d[0] * r[2], d[0] * r[1] - d[1] * r[0]};
+   const float mm[16] = {r[0], u[0], d[0], eye[0], r[1], u[1], d[1], eye[1], r[2], u[2], d[2], eye[2], 0, 0, 0, 1};
+   std::memcpy(m, mm, sizeof(mm));
+}
+
+}  // namespace
+}  // namespace ptx
+
+extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chosen) {
+   if (!dev || !scene) return set_error(PTX_ERR_INVALID, "NULL argument");
+   PTX_HIP_TRY(hipSetDevice(dev->ordinal));
+   const ptx_tuning saved = dev->tuning;
+   ptx_tuning best = saved;
+   // 1. traversal: LDS stack depth x refill threshold on 2^20 incoherent rays in the root box
+   const uint32_t n = 1u << 20;
+   std::vector<float> rays((size_t)n * 6);
+   uint64_t st = 0x9E3779B97F4A7C15ull;
+   auto rnd = [&]() {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      return (float)((st >> 40) * (1.0 / 16777216.0));
+   };
+   const float *bb = scene->d.root_box;
+   for (uint32_t i = 0; i < n; i++) {
+      float *r = &rays[6 * (size_t)i];
+      for (int k = 0; k < 3; k++) r[k] = bb[k] + (bb[3 + k] - bb[k]) * rnd();
+      float z = 2 * rnd() - 1, ph = 6.2831853f * rnd(), s = std::sqrt(std::max(0.f, 1 - z * z));
+      r[3] = s * std::cos(ph); r[4] = s * std::sin(ph); r[5] = z;
+   }
+   float *d_rays = nullptr;
+   ptx_hit *d_hits = nullptr;
+   PTX_HIP_TRY(hipMalloc((void **)&d_rays, rays.size() * sizeof(float)));
+   if (hipMalloc((void **)&d_hits, (size_t)n * sizeof(ptx_hit)) != hipSuccess) {
+      (void)hipFree(d_rays);
+      return set_error(PTX_ERR_OOM, "autotune buffers");
+   }
+   PTX_HIP_TRY(hipMemcpy(d_rays, rays.data(), rays.size() * sizeof(float), hipMemcpyHostToDevice));
+   hipEvent_t e0, e1;
+   (void)hipEventCreate(&e0);
+   (void)hipEventCreate(&e1);
+   float best_ms = 1e30f;
+   int rc = PTX_OK;
+   {
+      std::lock_guard<std::mutex> lock(dev->mu);
+      for (uint32_t lds : {8u, 12u, 16u})
+         for (uint32_t refill : {1u, 8u, 32u}) {
+            dev->tuning.lds_stack = lds;
+            dev->tuning.refill_lanes = refill;
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 3 && rc == PTX_OK; rep++) {
+               (void)hipEventRecord(e0, dev->stream);
+               rc = run_closest_hit(scene, d_rays, n, d_hits, nullptr, nullptr, dev->stream);
+               (void)hipEventRecord(e1, dev->stream);
+               (void)hipEventSynchronize(e1);
+               float ms = 0;
+               (void)hipEventElapsedTime(&ms, e0, e1);
+               if (rep) ms_min = std::min(ms_min, ms);
+            }
+            if (rc == PTX_OK && ms_min < best_ms) {
+               best_ms = ms_min;
+               best.lds_stack = lds;
+               best.refill_lanes = refill;
+            }
+         }
+   }
+   (void)hipFree(d_rays);
+   (void)hipFree(d_hits);
+   dev->tuning = best;
+   // 2. path capacity of the wavefront queues, on a probe render of the scene
+   if (rc == PTX_OK) {
+      ptx_job_desc job{};
+      const float c[3] = {(bb[0] + bb[3]) / 2, (bb[1] + bb[4]) / 2, (bb[2] + bb[5]) / 2};
+      const float ext = std::max({bb[3] - bb[0], bb[4] - bb[1], bb[5] - bb[2]});
+      const float eye[3] = {c[0] + 0.3f * ext, c[1] + 0.4f * ext, c[2] - 1.5f * ext};
+      look_at(eye, c, job.camera.camera_to_world);
+      job.camera.fov = 50.f;
+      job.camera.width = 256;
+      job.camera.height = 256;
+      job.camera.one_over_width = 1.f / 256.f;
+      job.camera.one_over_height = 1.f / 256.f;
+      job.camera.aspect_ratio = 1.f;
+      job.camera.tan_fov_over_2 = (float)std::tan(50.0 * 3.14159265358979 / 360.0);
+      job.spp = 32;
+      job.max_depth = 4;
+      job.seed = 1;
+      std::vector<float> film((size_t)256 * 256 * 3);
+      std::vector<ptx::RenderJob> rj(4);
+      for (auto &r : rj) { r.desc = job; r.scene_index = 0; r.npix = 256 * 256; }
+      std::vector<float *> films(4, film.data());
+      double best_s = 1e30;
+      uint32_t best_paths = best.paths_per_wave;
+      std::lock_guard<std::mutex> lock(dev->mu);
+      for (uint32_t paths : {1u << 20, 1u << 21, 1u << 22, 1u << 23}) {
+         dev->tuning.paths_per_wave = paths;
+         double t_min = 1e30;
+         for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
+            const double t0 = now_s();
+            rc = engine_of(dev)->run_batch(rj, {scene}, films);
+            t_min = std::min(t_min, now_s() - t0);
+         }
+         if (rc == PTX_OK && t_min < best_s) {
+            best_s = t_min;
+            best_paths = paths;
+         }
+      }
+      best.paths_per_wave = best_paths;
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   dev->tuning = rc == PTX_OK ? best : saved;
+   if (chosen) *chosen = dev->tuning;
+   return rc;
+}

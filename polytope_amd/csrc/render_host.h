@@ -14,6 +14,7 @@ struct RenderJob {
 };
 
 int validate_job(const ptx_job_desc *j);
+JobDev job_dev(const ptx_job_desc &j, uint32_t scene_index);
 // Device bytes a job holds while it runs (film + primary-hit records).
 size_t job_device_bytes(const ptx_job_desc &j);
 
@@ -47,6 +48,15 @@ struct RenderEngine {
    void release();
    static size_t bytes_for(uint32_t paths);
    int ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uint32_t nscenes, uint32_t nchunks);
+   // Allocates the job's film and primary records, zeroes the film, traces the
+   // primary hits.  J.film / J.primary are set.
+   int start_job(JobDev &J, const DevScene &S, uint32_t npix);
+   // One round over the chunks in d_round (a device copy of r); the job and
+   // scene tables must hold the round's jobs.
+   void round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth);
+   int finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host);
+   void free_job(JobDev &J);
+   uint32_t capacity_for(uint64_t total_paths, uint32_t max_npix) const;
    int run_batch(const std::vector<RenderJob> &jobs, const std::vector<const ptx_scene *> &scenes,
                  const std::vector<float *> &films_host);
    int run_plain(const ptx_scene *scene, const ptx_job_desc &desc, float *film_host);
