@@ -48,6 +48,13 @@ def parse():
     ap.add_argument("--refill", type=int, default=0, help="idle lanes that trigger a refill (0 = default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-coherent", action="store_true", help="skip the coherent-rate measurement")
+    ap.add_argument("--skip-render", action="store_true", help="skip the plain vs batched render comparison")
+    ap.add_argument("--skip-serving", action="store_true", help="skip the Poisson serving run")
+    ap.add_argument("--render-subdiv", type=int, default=8)
+    ap.add_argument("--render-instances", type=int, default=32)
+    ap.add_argument("--render-spp", type=int, default=64)
+    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-rate", type=float, default=8.0)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
@@ -83,6 +90,73 @@ def cpu_baseline(meshes, rays: np.ndarray, threads: int):
     dt = time.perf_counter() - t0
     return {"value": round(len(rays) / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {len(rays)} incoherent rays of the workload, oracle restatement, {threads} threads"}
+
+
+def render_section(dev, args):
+    """Plain (one job per launch series) vs batched (all jobs in shared wavefront
+    queues) samples/s on the 16-job instanced-icosphere workload."""
+    from polytope_amd import Scene, render, render_batch
+    from polytope_amd import scenes as S
+    t0 = time.perf_counter()
+    meshes, sky, jobs = S.batched_render_workload(args.render_subdiv, args.render_instances, 16, args.render_spp, 8)
+    scene = Scene(dev, meshes, sky)
+    info = scene.info()
+    print(f"[bench] render scene: {sum(m.num_faces for m in meshes)} triangles, BVH build "
+          f"{info['build_seconds']:.1f} s, {info['device_bytes'] / 1e9:.2f} GB on device "
+          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    samples = sum(c.width * c.height * spp for c, spp, _, _ in jobs)
+    batch = [(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
+    render_batch([(scene, jobs[0][0], 1, 8, 1)])                  # warm the engine and graphs
+    tb = time.perf_counter()
+    films_b = render_batch(batch)
+    t_batched = time.perf_counter() - tb
+    print(f"[bench] batched: {samples / t_batched / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
+    tp = time.perf_counter()
+    films_p = [render(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
+    t_plain = time.perf_counter() - tp
+    print(f"[bench] plain: {samples / t_plain / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(films_b, films_p))
+    scene.close()
+    return {"workload": f"icosphere subdiv {args.render_subdiv} x {args.render_instances + 1} instances + ground "
+                        f"({sum(m.num_faces for m in meshes)} tris), 16 jobs 256x256..1920x1080, "
+                        f"{args.render_spp} spp, depth 8, 70% Lambert / 30% mirror",
+            "samples": samples, "batched_s": round(t_batched, 3), "plain_s": round(t_plain, 3),
+            "batched_samples_per_s": round(samples / t_batched, 1), "plain_samples_per_s": round(samples / t_plain, 1),
+            "batched_vs_plain": round(t_plain / t_batched, 3), "films_identical": same,
+            "bvh_build_s": round(info["build_seconds"], 2)}
+
+
+def serving_section(dev, args):
+    """Poisson stream of render jobs through the continuous-batching server."""
+    from polytope_amd import Scene, Server
+    from polytope_amd import scenes as S
+    scene_defs, reqs = S.serving_requests(args.serving_requests, args.serving_rate)
+    scenes_ = [Scene(dev, m, sky) for m, sky in scene_defs]
+    srv = Server(dev, hbm_budget_bytes=64 << 30)
+    ids = []
+    t0 = time.perf_counter()
+    for arr, si, cam, spp, depth, seed in reqs:
+        delay = arr - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append((srv.submit(scenes_[si], cam, spp, depth, seed), cam.width * cam.height * spp))
+    lat, samples, first, last = [], 0, None, None
+    for jid, n in ids:
+        _, st = srv.wait(jid)
+        lat.append(st["end_time"] - st["submit_time"])
+        samples += n
+        first = st["submit_time"] if first is None else min(first, st["submit_time"])
+        last = st["end_time"] if last is None else max(last, st["end_time"])
+    srv.close()
+    for s_ in scenes_:
+        s_.close()
+    lat = np.array(lat)
+    return {"requests": len(reqs), "rate_hz": args.serving_rate,
+            "scenes_tris": [sum(m.num_faces for m in d[0]) for d in scene_defs],
+            "throughput_samples_per_s": round(samples / (last - first), 1),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p95_s": round(float(np.percentile(lat, 95)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4)}
 
 
 def main():
@@ -187,6 +261,13 @@ def main():
         "tuning": dev.get_tuning(),
         "cpu_baseline": cpu,
     }
+    sc.close()
+    del d_rays, hits, cnt
+    torch.cuda.empty_cache()
+    if not args.skip_render:
+        out["render"] = render_section(dev, args)
+    if not args.skip_serving:
+        out["serving"] = serving_section(dev, args)
     print(json.dumps(out), flush=True)
     if args.calibrate_copy_mb:
         src = torch.empty(args.calibrate_copy_mb << 20, dtype=torch.uint8, device="cuda")
@@ -194,7 +275,6 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(f"[bench] calibration copy of {args.calibrate_copy_mb} MiB done", file=sys.stderr, flush=True)
-    sc.close()
     dev.close()
 
 

@@ -83,13 +83,14 @@ def build_bvh(meshes: Sequence[MeshData], threads: int = 0) -> Tuple[np.ndarray,
     the host by the library's builder (ptx_bvh_build); needs no GPU."""
     lib = load()
     arr, keep = mesh_descs(meshes)
-    nn, np_ = C.c_uint32(0), C.c_uint32(0)
+    faces = sum(int(m.faces.shape[0]) for m in meshes)
+    nodes = np.zeros(max(1, 2 * faces - 1), BVH_NODE_DTYPE)      # a binary tree over F faces has <= 2F-1 nodes
+    pairs = np.zeros((max(1, faces), 2), np.uint32)
+    nn, np_ = C.c_uint32(nodes.shape[0]), C.c_uint32(pairs.shape[0])
     marr = C.cast(arr, C.POINTER(_abi.MeshDesc))
-    check(lib.ptx_bvh_build(marr, len(meshes), threads, None, C.byref(nn), None, C.byref(np_)))
-    nodes = np.zeros(nn.value, BVH_NODE_DTYPE)
-    pairs = np.zeros((np_.value, 2), np.uint32)
     check(lib.ptx_bvh_build(marr, len(meshes), threads, nodes.ctypes.data, C.byref(nn), pairs.ctypes.data,
                             C.byref(np_)))
+    nodes, pairs = nodes[:nn.value].copy(), pairs[:np_.value].copy()
     return nodes, pairs
 
 

@@ -18,6 +18,9 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
@@ -251,6 +254,14 @@ bool Builder::split(uint32_t ni, bool parallel) {
 }
 
 void Builder::run(HostBvh &out) {
+   const bool trace = std::getenv("PTX_BUILD_TRACE") != nullptr;
+   auto t_last = std::chrono::steady_clock::now();
+   auto mark = [&](const char *what) {
+      if (!trace) return;
+      const auto t = std::chrono::steady_clock::now();
+      std::fprintf(stderr, "[bvh] %s %.3f s\n", what, std::chrono::duration<double>(t - t_last).count());
+      t_last = t;
+   };
    size_t total = 0;
    for (const auto &m : meshes_) total += m.nf;
    info_.resize(total);
@@ -301,15 +312,18 @@ void Builder::run(HostBvh &out) {
    nodes_[0].bb = root;
    nodes_[0].lo = 0;
    nodes_[0].n = (uint32_t)total;
+   mark("face boxes");
 
-   // Phase 1: breadth-first over large nodes with parallel passes.
+   // Phase 1: breadth-first over the nodes too large to be one worker's subtree,
+   // each split with its per-face passes on all threads.
+   const size_t task_limit = std::max<size_t>(kTaskNode, total / (size_t(pool_.size()) * 16));
    std::vector<uint32_t> small;
    {
       std::vector<uint32_t> frontier = {0};
       while (!frontier.empty()) {
          std::vector<uint32_t> next;
          for (uint32_t ni : frontier) {
-            if (nodes_[ni].n <= kTaskNode || pool_.size() == 1) { small.push_back(ni); continue; }
+            if (nodes_[ni].n <= task_limit || pool_.size() == 1) { small.push_back(ni); continue; }
             if (split(ni, nodes_[ni].n >= kParallelNode)) {
                next.push_back((uint32_t)nodes_[ni].high);
                next.push_back((uint32_t)nodes_[ni].high + 1);
@@ -318,6 +332,9 @@ void Builder::run(HostBvh &out) {
          frontier.swap(next);
       }
    }
+   // largest subtrees first for load balance
+   std::sort(small.begin(), small.end(), [&](uint32_t a, uint32_t b) { return nodes_[a].n > nodes_[b].n; });
+   mark("large nodes");
    // Phase 2: whole subtrees per worker.
    {
       std::atomic<size_t> cursor{0};
@@ -344,57 +361,66 @@ void Builder::run(HostBvh &out) {
       for (auto &t : ts) t.join();
    }
 
-   // Flatten: DFS pre-order, low child first (bvh.cpp:652-693).
+   mark("subtrees");
+   // Flatten: DFS pre-order, low child first (bvh.cpp:652-693).  Children are
+   // allocated after their parent, so a reverse sweep gives subtree sizes and a
+   // forward sweep gives each child's DFS index from its parent's.
    const uint32_t used = next_node_.load();
    std::vector<uint32_t> subtree(used, 1);
-   {
-      // children always have larger indices than parents: reverse sweep = post-order sizes
-      for (int64_t i = (int64_t)used - 1; i >= 0; i--) {
-         const Node &nd = nodes_[(size_t)i];
-         if (nd.high >= 0) subtree[i] = 1 + subtree[nd.high] + subtree[nd.high + 1];
-      }
+   for (int64_t i = (int64_t)used - 1; i >= 0; i--) {
+      const Node &nd = nodes_[(size_t)i];
+      if (nd.high >= 0) subtree[i] = 1 + subtree[nd.high] + subtree[nd.high + 1];
    }
-   out.nodes.assign(total ? subtree[0] : 0, ptx_bvh_node{});
+   const uint32_t nn = total ? subtree[0] : 0;
+   out.nodes.assign(nn, ptx_bvh_node{});
    out.pairs.assign(2 * total, 0);
    if (!total) return;
-   struct Item { uint32_t node, dfs; };
-   std::vector<Item> order;           // leaves in DFS order with their DFS index
-   order.reserve(used / 2 + 1);
-   std::vector<Item> stack = {{0, 0}};
-   while (!stack.empty()) {
-      const Item it = stack.back();
-      stack.pop_back();
-      const Node &nd = nodes_[it.node];
-      ptx_bvh_node &cn = out.nodes[it.dfs];
-      for (int k = 0; k < 3; k++) { cn.p0[k] = nd.bb.lo[k]; cn.p1[k] = nd.bb.hi[k]; }
-      cn.flags = nd.axis;
-      if (nd.high < 0) {
-         cn.num_faces = (uint16_t)nd.n;
-         order.push_back(it);
-      } else {
-         const uint32_t low = (uint32_t)nd.high + 1, high = (uint32_t)nd.high;
-         cn.offset = subtree[low] + 1;
-         cn.num_faces = 0;
-         stack.push_back({high, it.dfs + 1 + subtree[low]});
-         stack.push_back({low, it.dfs + 1});
-      }
+   std::vector<uint32_t> dfs(used, 0);
+   for (uint32_t i = 0; i < used; i++) {
+      const Node &nd = nodes_[i];
+      if (nd.high < 0) continue;
+      dfs[nd.high + 1] = dfs[i] + 1;                             // low child
+      dfs[nd.high] = dfs[i] + 1 + subtree[nd.high + 1];         // high child
    }
-   // leaf offsets: prefix sum of leaf sizes in DFS order; pairs sorted with the
-   // reference's comparator (bvh.cpp:666-671) by the same std::sort.
-   std::vector<uint32_t> offs(order.size());
+   // faces per DFS position (0 for interior nodes), prefix-summed in DFS order
+   std::vector<uint32_t> leaf_node(nn, 0xFFFFFFFFu);
+   std::vector<uint32_t> offs(nn, 0);
+   pool_.chunks(used, pool_.size(), [&](unsigned, size_t a, size_t b) {
+      for (size_t i = a; i < b; i++) {
+         const Node &nd = nodes_[i];
+         ptx_bvh_node &cn = out.nodes[dfs[i]];
+         for (int k = 0; k < 3; k++) { cn.p0[k] = nd.bb.lo[k]; cn.p1[k] = nd.bb.hi[k]; }
+         cn.flags = nd.axis;
+         if (nd.high < 0) {
+            cn.num_faces = (uint16_t)nd.n;
+            leaf_node[dfs[i]] = (uint32_t)i;
+         } else {
+            cn.offset = subtree[nd.high + 1] + 1;
+            cn.num_faces = 0;
+         }
+      }
+   });
    {
       uint64_t acc = 0;
-      for (size_t i = 0; i < order.size(); i++) {
-         offs[i] = (uint32_t)acc;
-         acc += nodes_[order[i].node].n;
+      for (uint32_t d = 0; d < nn; d++) {
+         offs[d] = (uint32_t)acc;
+         if (leaf_node[d] != 0xFFFFFFFFu) acc += nodes_[leaf_node[d]].n;
       }
    }
+   struct Item { uint32_t node, dfs; };
+   std::vector<Item> order;           // leaves in DFS order
+   order.reserve(used / 2 + 1);
+   for (uint32_t d = 0; d < nn; d++)
+      if (leaf_node[d] != 0xFFFFFFFFu) order.push_back({leaf_node[d], d});
+   mark("dfs order");
+   // pairs sorted with the reference's comparator (bvh.cpp:666-671) by the same std::sort
    const unsigned nch = order.size() > 4096 ? pool_.size() : 1;
    pool_.chunks(order.size(), nch, [&](unsigned, size_t a, size_t b) {
       std::vector<std::pair<unsigned int, unsigned int>> leaf;
       for (size_t i = a; i < b; i++) {
          const Node &nd = nodes_[order[i].node];
-         out.nodes[order[i].dfs].offset = offs[i];
+         const uint32_t off = offs[order[i].dfs];
+         out.nodes[order[i].dfs].offset = off;
          leaf.clear();
          for (uint32_t k = 0; k < nd.n; k++) {
             const uint32_t g = idx_[nd.lo + k];
@@ -407,11 +433,12 @@ void Builder::run(HostBvh &out) {
                       return lhs.first < lhs.second;
                    });
          for (uint32_t k = 0; k < nd.n; k++) {
-            out.pairs[2 * (size_t(offs[i]) + k)] = leaf[k].first;
-            out.pairs[2 * (size_t(offs[i]) + k) + 1] = leaf[k].second;
+            out.pairs[2 * (size_t(off) + k)] = leaf[k].first;
+            out.pairs[2 * (size_t(off) + k) + 1] = leaf[k].second;
          }
       }
    });
+   mark("leaves");
 }
 
 }  // namespace

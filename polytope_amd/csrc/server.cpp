@@ -113,7 +113,7 @@ void ptx_server::loop() {
                continue;
             }
             const size_t need = job_device_bytes(j->desc);
-            if (cfg.max_jobs_in_flight && running.size() >= cfg.max_jobs_in_flight) break;
+            if (cfg.max_jobs_in_flight && running.size() + admitted.size() >= cfg.max_jobs_in_flight) break;
             if (used_bytes + need > cfg.hbm_budget_bytes - pool_bytes) break;
             auto sit = scene_slots.find(j->scene);
             if (sit == scene_slots.end() && scene_slots.size() >= kMaxServerScenes) break;

@@ -414,3 +414,68 @@ def plain_render_scene(seed: int = 1, width: int = 640, height: int = 480):
         meshes.append(MeshData(sv, sf, object_to_world=o2w.astype(np.float32), refl=refl))
     cam = Camera(look_at((0.0, 2.2, -6.0), (0.0, 0.8, 1.5)), 60.0, width, height)
     return meshes, cam, (180.0, 200.0, 230.0)
+
+
+def batched_render_workload(subdiv: int = 8, instances: int = 32, jobs: int = 16, spp: int = 64,
+                            max_depth: int = 8, seed: int = 3):
+    """Batched wavefront workload: a displaced icosphere (the shared geometry of
+    ``instances`` + 1 meshes: the original plus instanced copies under seeded rigid
+    transforms) over a 2-triangle ground plane; 70 % of the meshes Lambertian
+    (albedo U[0.2, 0.8]^3), 30 % perfect mirrors; ``jobs`` render jobs whose
+    resolutions step from 256x256 to 1920x1080, ``spp`` samples, depth ``max_depth``,
+    seeds seed..seed+jobs-1.  Returns (meshes, skybox, [(camera, spp, depth, seed)])."""
+    v, f = icosphere(subdiv, radius=1.0, noise=0.05, seed=seed)
+    rng = Rng(seed)
+    u = rng.uniform(4 * (instances + 1))
+    meshes = []
+    for i in range(instances + 1):
+        o2w = IDENTITY.copy() if i == 0 else rigid(1000 + i, 9.0)
+        o2w = o2w.copy()
+        o2w[1, 3] = abs(float(o2w[1, 3])) + 1.0          # keep the copies above the ground
+        mirror = u[4 * i] < 0.3
+        refl = tuple(float(0.2 + 0.6 * u[4 * i + 1 + k]) for k in range(3))
+        meshes.append(MeshData(v, f, object_to_world=o2w.astype(np.float32),
+                               brdf=MIRROR if mirror else LAMBERT, refl=(0.9, 0.9, 0.9) if mirror else refl))
+    gv, gf = quad(40.0, 0.0)
+    meshes.append(MeshData(gv, gf, refl=(0.5, 0.5, 0.5)))
+    job_list = []
+    for j in range(jobs):
+        t = j / max(1, jobs - 1)
+        w = int(round(256 + (1920 - 256) * t))
+        h = int(round(256 + (1080 - 256) * t))
+        ang = 2 * math.pi * j / jobs
+        s_, c_ = _sincos64(np.array([ang]))
+        eye = (float(22.0 * c_[0]), 9.0, float(22.0 * s_[0]))
+        cam = Camera(look_at(eye, (0.0, 2.0, 0.0)), 55.0, w, h)
+        job_list.append((cam, spp, max_depth, seed + j))
+    return meshes, (150.0, 170.0, 200.0), job_list
+
+
+def serving_requests(n: int, rate_hz: float, seed: int = 4, subdivs=(5, 6, 7, 8)):
+    """Poisson request stream for the render service: ``n`` jobs at rate ``rate_hz``
+    over displaced-icosphere scenes (subdivisions ``subdivs``: 20k to 1.3M faces),
+    resolution between 256x256 and 1920x1080 (log-uniform area), spp log-uniform in
+    [4, 256], depth uniform in 4..12.  Returns (scenes, [(arrival_s, scene_index,
+    camera, spp, depth, seed)])."""
+    scenes_ = []
+    for i, sd in enumerate(subdivs):
+        v, f = icosphere(sd, radius=1.0, noise=0.05, seed=seed + i)
+        gv, gf = quad(6.0, -1.05)
+        scenes_.append(([MeshData(v, f, refl=(0.7, 0.6, 0.5)),
+                         MeshData(gv, gf, brdf=MIRROR if i % 2 else LAMBERT, refl=(0.6, 0.6, 0.6))],
+                        (140.0, 160.0, 190.0)))
+    rng = Rng(seed)
+    u = [rng.uniform(n) for _ in range(6)]
+    gaps = -np.log(np.maximum(u[0], 1e-12)) / rate_hz      # exponential inter-arrival times
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    for i in range(n):
+        area = math.exp(math.log(256 * 256) + (math.log(1920 * 1080) - math.log(256 * 256)) * u[1][i])
+        w = int(min(1920, max(256, round(math.sqrt(area * 16 / 9)))))
+        h = int(min(1080, max(256, round(area / w))))
+        spp = int(round(math.exp(math.log(4) + (math.log(256) - math.log(4)) * u[2][i])))
+        depth = 4 + int(u[3][i] * 9) % 9
+        sc = int(u[4][i] * len(subdivs)) % len(subdivs)
+        cam = Camera(look_at((0.4, 0.6, -3.2), (0.0, 0.0, 0.0)), 50.0, w, h)
+        reqs.append((float(arrivals[i]), sc, cam, spp, depth, seed + 100 + i))
+    return scenes_, reqs
