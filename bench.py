@@ -93,9 +93,11 @@ def cpu_baseline(meshes, rays: np.ndarray, threads: int):
 
 
 def render_section(dev, args):
-    """Plain (one job per launch series) vs batched (all jobs in shared wavefront
-    queues) samples/s on the 16-job instanced-icosphere workload."""
+    """Plain (one job per launch series) vs batched (all jobs' samples in shared
+    rounds: path regeneration, and the wavefront form for comparison) samples/s
+    on the 16-job instanced-icosphere workload."""
     from polytope_amd import Scene, render, render_batch
+    from polytope_amd import _abi
     from polytope_amd import scenes as S
     t0 = time.perf_counter()
     meshes, sky, jobs = S.batched_render_workload(args.render_subdiv, args.render_instances, 16, args.render_spp, 8)
@@ -111,18 +113,27 @@ def render_section(dev, args):
     films_b = render_batch(batch)
     t_batched = time.perf_counter() - tb
     print(f"[bench] batched: {samples / t_batched / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
+    dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT)
+    render_batch([(scene, jobs[0][0], 1, 8, 1)])
+    tw = time.perf_counter()
+    films_w = render_batch(batch)
+    t_wave = time.perf_counter() - tw
+    dev.set_tuning()
+    print(f"[bench] batched wavefront: {samples / t_wave / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
     tp = time.perf_counter()
     films_p = [render(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
     t_plain = time.perf_counter() - tp
     print(f"[bench] plain: {samples / t_plain / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
-    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(films_b, films_p))
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) and
+               np.array_equal(a.view(np.uint32), w.view(np.uint32)) for a, b, w in zip(films_b, films_p, films_w))
     scene.close()
     return {"workload": f"icosphere subdiv {args.render_subdiv} x {args.render_instances + 1} instances + ground "
                         f"({sum(m.num_faces for m in meshes)} tris), 16 jobs 256x256..1920x1080, "
                         f"{args.render_spp} spp, depth 8, 70% Lambert / 30% mirror",
             "samples": samples, "batched_s": round(t_batched, 3), "plain_s": round(t_plain, 3),
             "batched_samples_per_s": round(samples / t_batched, 1), "plain_samples_per_s": round(samples / t_plain, 1),
-            "batched_vs_plain": round(t_plain / t_batched, 3), "films_identical": same,
+            "batched_vs_plain": round(t_plain / t_batched, 3),
+            "wavefront_samples_per_s": round(samples / t_wave, 1), "films_identical": same,
             "bvh_build_s": round(info["build_seconds"], 2)}
 
 

@@ -166,18 +166,26 @@ int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t
 /* Plain GPU render of one job (one launch series per bounce, as the reference
  * CUDA path), film_rgb = W*H*3 floats on the host (BoxFilter::Output values). */
 int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb);
-/* Batched render: all jobs' paths merged into shared wavefront queues; each
- * bounce is one captured hipGraph replay.  films[i] = W_i*H_i*3 host floats. */
+/* Batched render: all jobs' samples are merged into rounds of path slots that
+ * share the GPU.  By default a round is one persistent path-regeneration launch
+ * (every lane takes the next path slot as soon as its path ends); the wavefront
+ * form keeps the paths in queues and replays one captured hipGraph per bounce.
+ * films[i] = W_i*H_i*3 host floats. */
 int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_t n_jobs,
                      float *const *films);
 
-/* Tuning knobs shared by ptx_render_batch and the server; zero = autotuned. */
+/* Round form of ptx_render_batch and the server (ptx_tuning.render_mode). */
+enum { PTX_RENDER_AUTO = 0, PTX_RENDER_REGEN = 1, PTX_RENDER_WAVEFRONT = 2 };
+
+/* Tuning knobs shared by ptx_render_batch and the server; zero = default. */
 typedef struct ptx_tuning {
-   uint32_t paths_per_wave;  /* path slots in the wavefront queues                    */
+   uint32_t paths_per_wave;  /* path slots per round                                  */
    uint32_t block_size;      /* threads per block of the traversal kernel (256)       */
    uint32_t lds_stack;       /* LDS stack entries per lane: 8, 12, 16 or 24           */
    uint32_t refill_lanes;    /* idle lanes that trigger a wave's ray refill (1..64)   */
    uint32_t blocks_per_cu;   /* persistent blocks per CU; 0 = occupancy limit         */
+   uint32_t render_mode;     /* PTX_RENDER_*: AUTO = regeneration where a round fits  */
+                             /* its LDS tables (64 chunks, 64 scenes), else wavefront */
 } ptx_tuning;
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);

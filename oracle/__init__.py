@@ -55,6 +55,9 @@ def lib() -> C.CDLL:
         L = C.CDLL(str(LIB))
         L.orc_scene_create.restype = C.c_void_p
         L.orc_scene_create.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+        L.orc_scene_create_prebuilt.restype = C.c_void_p
+        L.orc_scene_create_prebuilt.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                C.c_void_p, C.c_uint32]
         L.orc_scene_destroy.argtypes = [C.c_void_p]
         L.orc_num_nodes.restype = C.c_uint32
         L.orc_num_nodes.argtypes = [C.c_void_p]
@@ -81,12 +84,21 @@ class OrcJob(C.Structure):
 class OracleScene:
     """The oracle's scene: world vertices, the restated reference BVH."""
 
-    def __init__(self, meshes, skybox: Optional[Sequence[float]] = None):
+    def __init__(self, meshes, skybox: Optional[Sequence[float]] = None, bvh=None):
+        """bvh: optional (nodes, pairs) in the reference compact layout to use
+        instead of building the restated tree."""
         from polytope_amd import mesh_descs
         arr, keep = mesh_descs(meshes)
         sky = (C.c_float * 3)(*(skybox if skybox is not None else (0.0, 0.0, 0.0)))
-        self.h = lib().orc_scene_create(C.cast(arr, C.c_void_p), len(meshes), 1 if skybox is not None else 0,
-                                        C.cast(sky, C.c_void_p))
+        if bvh is None:
+            self.h = lib().orc_scene_create(C.cast(arr, C.c_void_p), len(meshes), 1 if skybox is not None else 0,
+                                            C.cast(sky, C.c_void_p))
+        else:
+            nodes = np.ascontiguousarray(bvh[0], NODE_DTYPE)
+            pairs = np.ascontiguousarray(bvh[1], np.uint32).reshape(-1, 2)
+            self.h = lib().orc_scene_create_prebuilt(C.cast(arr, C.c_void_p), len(meshes),
+                                                     1 if skybox is not None else 0, C.cast(sky, C.c_void_p),
+                                                     nodes.ctypes.data, len(nodes), pairs.ctypes.data, len(pairs))
         self._keep = (arr, keep)
 
     def bvh(self) -> Tuple[np.ndarray, np.ndarray]:

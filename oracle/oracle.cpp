@@ -757,6 +757,44 @@ orc::Scene* orc_scene_create(const orc::MeshIn* meshes, uint32_t n_meshes, int h
    return s;
 }
 
+// Same scene with a BVH supplied by the caller (reference compact layout:
+// 32-byte nodes and (mesh, face) leaf pairs) instead of building one.  Lets a
+// test check traversal on scenes whose build would take the single-threaded
+// restatement minutes, with the builder's parity checked separately.
+orc::Scene* orc_scene_create_prebuilt(const orc::MeshIn* meshes, uint32_t n_meshes, int has_sky, const float* sky,
+                                      const void* nodes, uint32_t num_nodes, const uint32_t* pairs,
+                                      uint32_t num_pairs) {
+   auto* s = new orc::Scene();
+   s->meshes.resize(n_meshes);
+   for (uint32_t mi = 0; mi < n_meshes; mi++) {
+      const orc::MeshIn& in = meshes[mi];
+      orc::Mesh& m = s->meshes[mi];
+      m.nf = in.num_faces;
+      m.wv.resize(size_t(3) * in.num_faces);
+      m.has_normals = in.nx != nullptr;
+      if (m.has_normals) m.vn.resize(size_t(3) * in.num_faces);
+      for (uint32_t f = 0; f < in.num_faces; f++) {
+         const uint32_t ix[3] = {in.fv0[f], in.fv1[f], in.fv2[f]};
+         for (int c = 0; c < 3; c++) {
+            m.wv[3 * f + c] = orc::apply_point_inplace(in.object_to_world, {in.x[ix[c]], in.y[ix[c]], in.z[ix[c]]});
+            if (m.has_normals)
+               m.vn[3 * f + c] = orc::apply_normal(in.world_to_object, {in.nx[ix[c]], in.ny[ix[c]], in.nz[ix[c]]});
+         }
+      }
+      m.brdf = in.brdf;
+      std::memcpy(m.refl, in.refl, sizeof(m.refl));
+      m.light = in.is_light != 0;
+      std::memcpy(m.L, in.emission, sizeof(m.L));
+   }
+   s->has_sky = has_sky != 0;
+   if (sky) std::memcpy(s->sky, sky, sizeof(s->sky));
+   s->nodes.resize(num_nodes);
+   std::memcpy(s->nodes.data(), nodes, size_t(num_nodes) * sizeof(orc::CompactNode));
+   s->pairs.resize(num_pairs);
+   for (uint32_t i = 0; i < num_pairs; i++) s->pairs[i] = {pairs[2 * i], pairs[2 * i + 1]};
+   return s;
+}
+
 void orc_scene_destroy(orc::Scene* s) { delete s; }
 uint32_t orc_num_nodes(const orc::Scene* s) { return (uint32_t)s->nodes.size(); }
 uint32_t orc_num_pairs(const orc::Scene* s) { return (uint32_t)s->pairs.size(); }

@@ -264,6 +264,7 @@ int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
    if (!dev || !t) return set_error(PTX_ERR_INVALID, "NULL argument");
    if (t->lds_stack && t->lds_stack != 8 && t->lds_stack != 12 && t->lds_stack != 16 && t->lds_stack != 24)
       return set_error(PTX_ERR_INVALID, "lds_stack must be 0, 8, 12, 16 or 24");
+   if (t->render_mode > PTX_RENDER_WAVEFRONT) return set_error(PTX_ERR_INVALID, "render_mode must be 0, 1 or 2");
    dev->tuning = *t;
    return PTX_OK;
 }

@@ -30,6 +30,14 @@ void launch_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs
                        hipStream_t st);
 void launch_finalize(float *film, uint32_t npix, uint32_t spp, hipStream_t st);
 void launch_zero_floats(float *a, size_t n, hipStream_t st);
+int paths_blocks_per_cu();
+// Path-regeneration mode: one launch per round; rounds limited to
+// paths_max_chunks() chunks and paths_max_scenes() scenes.
+uint32_t paths_max_chunks();
+uint32_t paths_max_scenes();
+void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
+                  uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t grid,
+                  hipStream_t st);
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st);
 
 }  // namespace ptx

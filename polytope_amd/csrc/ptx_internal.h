@@ -79,6 +79,10 @@ struct Chunk {
    uint32_t job, s0, k, npix, base;
 };
 
+// Largest number of chunks in a round of the path-regeneration kernel (its
+// chunk table lives in LDS).
+constexpr uint32_t kMaxRoundChunks = 64;
+
 // ---------------------------------------------------------------- host-side scene
 
 struct HostBvh {

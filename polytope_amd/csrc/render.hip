@@ -3,19 +3,24 @@
 // with CenterSampler pixel centres (src/cpu/samplers/samplers.cpp:13-18), the
 // pinhole camera (src/cpu/cameras/PerspectiveCamera.cpp:27-49), Lambert and mirror
 // BRDFs (src/cpu/shading/brdf/*.h) and BoxFilter accumulation
-// (src/cpu/filters/BoxFilter.cpp:14-71), in two forms:
+// (src/cpu/filters/BoxFilter.cpp:14-71), in three forms:
 //
 //   plain     k_mega: one launch per sample, one thread per pixel path with every
 //             bounce inline — the launch shape of the reference CUDA path
 //             (PathTracerKernel::Trace, src/cuda/kernels/path_tracer.cu:899-1018);
-//   wavefront k_primary traces each pixel's primary ray once (CenterSampler gives
-//             every sample of a pixel the same ray), k_round_start shades the
-//             primary hit per sample, then per bounce k_extend (the persistent
-//             traversal of traverse.h fed from a queue) and k_shade (frame, BRDF,
-//             compaction into the next queue); k_accumulate adds each pixel's
-//             samples to its film in sample order and k_finalize scales by 1/spp.
+//   regen     k_primary traces each pixel's primary ray once (CenterSampler gives
+//             every sample of a pixel the same ray); then one persistent k_paths
+//             launch per round of path slots (many jobs' samples) in which every
+//             lane carries a path from its cached primary hit through all its
+//             bounces and takes the next slot as soon as it ends;
+//   wavefront k_primary as above, k_round_start shades the primary hit per
+//             sample, then per bounce k_extend (the persistent traversal of
+//             traverse.h fed from a queue) and k_shade (frame, BRDF, compaction
+//             into the next queue).
+// Batched rounds end with k_accumulate, which adds each pixel's samples to its
+// film in sample order; k_finalize scales by 1/spp.
 //
-// Both forms run the same arithmetic in the same order, so their films are
+// All forms run the same arithmetic in the same order, so their films are
 // identical to each other and to the CPU restatement.
 #include <hip/hip_runtime.h>
 
@@ -571,6 +576,235 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
    }
 }
 
+
+// ---------------------------------------------------------------- path regeneration
+
+// One persistent launch per round: each lane carries a whole path — the cached
+// primary hit, then trace and shade for every bounce — and takes the next path
+// slot of the round as soon as its path ends, so no lane waits for the longest
+// path of its wave and small jobs share the GPU with large ones.  Each loop
+// step issues one mode's loads for every lane before any compute (the traversal
+// step's node or faces, the primary record, the hit's mesh index, its mesh
+// record, its faces), as in the closest-hit loop.  The round's chunk table and
+// scene table sit in LDS.
+enum : uint32_t { kPIdle = 0, kPPrim = 1, kPTrace = 2, kPShade1 = 3, kPShade2 = 4, kPShade3 = 5 };
+constexpr uint32_t kPModeMask = 7u, kPDbg = 8u;
+constexpr uint32_t kMaxRoundScenes = 64;
+
+struct LdsChunk {
+   uint32_t base, npix, s0, width;
+   uint32_t seed, max_depth, scene, pad;
+   const uint32_t *primary;
+   const void *pad2;
+};
+
+__device__ __forceinline__ uint32_t lds_find_chunk(const LdsChunk *c, uint32_t n, uint32_t slot) {
+   uint32_t lo = 0, hi = n;
+   while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (c[mid].base <= slot) lo = mid; else hi = mid;
+   }
+   return lo;
+}
+
+// Starts the traversal of ray s.r in scene S; returns the next mode, or kPIdle
+// after writing the miss result when the root box is missed.
+__device__ __forceinline__ uint32_t p_root(const DevScene &S, Trav &s, float *res, uint32_t slot,
+                                           const float src[3]) {
+   begin(s, S.root_box, S.root_ref, S.root_count);
+   if (s.cur != kNone) return kPTrace;
+   write_res(res, slot, miss_radiance(S, src));
+   return kPIdle;
+}
+
+template <int kLds>
+__global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots,
+                                               const JobDev *jobs, const DevScene *scenes, uint32_t nscenes,
+                                               float *res, uint32_t *work, uint4 *spill, uint32_t max_stack) {
+   __shared__ uint32_t lds_ref[kLds * 256];
+   __shared__ float lds_near[kLds * 256];
+   __shared__ uint16_t lds_cnt[kLds * 256];
+   __shared__ LdsChunk sch[kMaxRoundChunks];
+   __shared__ DevScene ssc[kMaxRoundScenes];
+   for (uint32_t i = threadIdx.x; i < nchunks; i += blockDim.x) {
+      const Chunk c = chunks[i];
+      const JobDev &J = jobs[c.job];
+      LdsChunk l;
+      l.base = c.base; l.npix = c.npix; l.s0 = c.s0; l.width = J.width;
+      l.seed = J.seed; l.max_depth = J.max_depth; l.scene = J.scene; l.pad = 0;
+      l.primary = J.primary; l.pad2 = nullptr;
+      sch[i] = l;
+   }
+   for (uint32_t i = threadIdx.x; i < nscenes; i += blockDim.x) ssc[i] = scenes[i];
+   __syncthreads();
+   const uint32_t lane = __lane_id();
+   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
+                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, max_stack};
+   RaySource srcq{nslots, blockIdx.x % kSegments, 0u, 0u, 0u, 0u, nslots == 0};
+   if (!srcq.done) srcq.request(work, lane);
+   Trav s;
+   s.hit = kNone;
+   s.after = 0;
+   s.cur = kNone;
+   s.cnt = 0;
+   s.sp = 0;
+   uint32_t mode = kPIdle;       // state | kPDbg (the reference's debug pixel)
+   uint32_t slot = 0, ci = 0, b = 0;
+   uint32_t t0 = 0;              // Shade1 -> Shade2: mesh; Shade2 -> Shade3: brdf
+   float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2 -> Shade3: reflectance
+   float src[3] = {1.f, 1.f, 1.f};
+   const Node64 *nodes = nullptr;
+   const float *tris = nullptr;
+   while (true) {
+      uint64_t idle = __ballot((mode & kPModeMask) == kPIdle);
+      if (idle && !srcq.done) {
+         while (idle) {
+            srcq.refill(work, lane);
+            if (srcq.done) break;
+            const uint32_t take = min((uint32_t)__popcll(idle), srcq.pool_left);
+            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+            const bool mine = ((idle >> lane) & 1ull) && rank < take;
+            if (mine) {
+               slot = srcq.pool_base + rank;
+               mode = kPPrim;
+            }
+            idle &= ~__ballot(mine);
+            srcq.pool_base += take;
+            srcq.pool_left -= take;
+         }
+      }
+      const uint32_t m = mode & kPModeMask;
+      if (!__any(m != kPIdle)) break;
+      // ------------------------------------------------------------ fetch
+      Fetched f;
+      if (m == kPTrace) {
+         fetch(s, nodes, tris, f);
+      } else if (m == kPPrim) {
+         ci = lds_find_chunk(sch, nchunks, slot);
+         const LdsChunk &c = sch[ci];
+         const uint32_t pixel = (slot - c.base) % c.npix;
+         const float4 *pr = reinterpret_cast<const float4 *>(c.primary + 16 * (size_t)pixel);
+         const float4 q0 = pr[0], q1 = pr[1], q2 = pr[2], q3 = pr[3];
+         f.q[0] = q0.x; f.q[1] = q0.y; f.q[2] = q0.z; f.q[3] = q0.w;
+         f.q[4] = q1.x; f.q[5] = q1.y; f.q[6] = q1.z; f.q[7] = q1.w;
+         f.q[8] = q2.x; f.q[9] = q2.y; f.q[10] = q2.z; f.q[11] = q2.w;
+         f.q[12] = q3.x; f.q[13] = q3.y; f.q[14] = q3.z; f.q[15] = q3.w;
+      } else if (m == kPShade1) {
+         f.q[0] = __uint_as_float(ssc[sch[ci].scene].pairs[2 * (size_t)s.hit]);
+      } else if (m == kPShade2) {
+         const uint32_t *mr = reinterpret_cast<const uint32_t *>(ssc[sch[ci].scene].meshes + t0);
+#pragma unroll
+         for (int k = 0; k < 9; k++) f.q[k] = __uint_as_float(mr[k]);
+         const float *v = tris + 9 * (size_t)s.hit;
+#pragma unroll
+         for (int k = 0; k < 9; k++) f.q[9 + k] = v[k];
+      } else if (m == kPShade3) {
+         const float *v = tris + 9 * (size_t)s.hit;
+         const float *nrm = ssc[sch[ci].scene].normals + 9 * (size_t)s.hit;
+#pragma unroll
+         for (int k = 0; k < 9; k++) { f.q[k] = v[k]; f.q[9 + k] = nrm[k]; }
+      }
+      // ------------------------------------------------------------ compute
+      if (m == kPTrace) {
+         if (!advance<kLds>(s, f, st, (mode & kPDbg) != 0)) {
+            if (s.hit == kNone) {
+               write_res(res, slot, miss_radiance(ssc[sch[ci].scene], src));
+               mode = kPIdle;
+            } else {
+               mode = (mode & kPDbg) | kPShade1;
+            }
+         }
+      } else if (m == kPPrim) {
+         const LdsChunk &c = sch[ci];
+         const uint32_t local = slot - c.base;
+         const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
+         const uint32_t w0 = __float_as_uint(f.q[0]);
+         if (w0 & 1u) {
+            write_res(res, slot, Radiance{f.q[1], f.q[2], f.q[3]});
+            mode = kPIdle;
+         } else {
+            const DevScene &S = ssc[c.scene];
+            const MeshRec &M = S.meshes[w0 >> 2];
+            Frame fr;
+            fr.lx = f.q[1]; fr.ly = f.q[2]; fr.lz = f.q[3];
+            fr.t1x = f.q[4]; fr.t1y = f.q[5]; fr.t1z = f.q[6];
+            fr.nx = f.q[7]; fr.ny = f.q[8]; fr.nz = f.q[9];
+            fr.t2x = f.q[10]; fr.t2y = f.q[11]; fr.t2z = f.q[12];
+            const float lin[3] = {f.q[13], f.q[14], f.q[15]};
+            src[0] = src[1] = src[2] = 1.f;
+            float wout[3];
+            bounce(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
+            if (src_is_zero(src)) {
+               write_res(res, slot, Radiance{0.f, 0.f, 0.f});
+               mode = kPIdle;
+            } else {
+               const uint32_t x = pixel % c.width, y = pixel / c.width;
+               const uint32_t dbg = (x == 128 && y == 128) ? kPDbg : 0u;
+               make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
+               nodes = S.nodes;
+               tris = S.tris;
+               b = 1;
+               mode = dbg | p_root(S, s, res, slot, src);
+               if ((mode & kPModeMask) == kPIdle) mode = kPIdle;
+            }
+         }
+      } else if (m == kPShade1) {
+         t0 = __float_as_uint(f.q[0]);
+         mode = (mode & kPDbg) | kPShade2;
+      } else if (m == kPShade2 || m == kPShade3) {
+         const LdsChunk &c = sch[ci];
+         bool go = true;
+         if (m == kPShade2) {
+            const int32_t brdf = __float_as_int(f.q[0]);
+            const int32_t is_light = __float_as_int(f.q[4]);
+            const uint32_t has_normals = __float_as_uint(f.q[8]);
+            if (is_light) {
+               write_res(res, slot, Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
+                                             (f.q[7] + 0.f) * src[2]});
+               mode = kPIdle;
+               go = false;
+            } else if (b >= c.max_depth) {
+               write_res(res, slot, Radiance{0.f, 0.f, 0.f});
+               mode = kPIdle;
+               go = false;
+            } else if (has_normals && ssc[c.scene].normals) {
+               t0 = (uint32_t)brdf;
+               t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
+               mode = (mode & kPDbg) | kPShade3;
+               go = false;
+            } else {
+               t0 = (uint32_t)brdf;
+               t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
+#pragma unroll
+               for (int k = 0; k < 9; k++) f.q[k] = f.q[9 + k];
+            }
+         }
+         if (go) {
+            MeshRec M;
+            M.brdf = (int32_t)t0;
+            M.refl[0] = t1; M.refl[1] = t2; M.refl[2] = t3;
+            Frame fr;
+            hit_frame_sel(&f.q[0], &f.q[9], m == kPShade3, s.r, s.hu, s.hv, s.hw, s.after, fr);
+            const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
+            const uint32_t local = slot - c.base;
+            const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
+            float lin[3], wout[3];
+            world_to_local(fr, d, lin);
+            bounce(M, fr, lin, pixel, sample, b, c.seed, wout, src);
+            if (src_is_zero(src)) {
+               write_res(res, slot, Radiance{0.f, 0.f, 0.f});
+               mode = kPIdle;
+            } else {
+               make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
+               b++;
+               const uint32_t dbg = mode & kPDbg;
+               mode = dbg | p_root(ssc[c.scene], s, res, slot, src);
+               if ((mode & kPModeMask) == kPIdle) mode = kPIdle;
+            }
+         }
+      }
+   }
+}
 }  // namespace dev
 
 // ---------------------------------------------------------------- launchers
@@ -629,6 +863,23 @@ void launch_finalize(float *film, uint32_t npix, uint32_t spp, hipStream_t st) {
 void launch_zero_floats(float *a, size_t n, hipStream_t st) {
    const uint32_t grid = (uint32_t)std::min<size_t>(4096, (n + 255) / 256);
    hipLaunchKernelGGL(dev::k_zero_floats, dim3(std::max(1u, grid)), dim3(256), 0, st, a, n);
+}
+
+int paths_blocks_per_cu() {
+   int b = 0;
+   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_paths<kRenderLds>, 256, 0);
+   return e == hipSuccess && b > 0 ? b : 1;
+}
+
+uint32_t paths_max_chunks() { return kMaxRoundChunks; }
+uint32_t paths_max_scenes() { return dev::kMaxRoundScenes; }
+
+void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
+                  uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t grid,
+                  hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
+   hipLaunchKernelGGL(dev::k_paths<kRenderLds>, dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, scenes,
+                      nscenes, res, work, spill, max_stack);
 }
 
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {

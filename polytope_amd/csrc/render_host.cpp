@@ -1,15 +1,17 @@
 // Host side of rendering: the plain one-job-per-launch-series path
-// (ptx_render), the batched wavefront path (ptx_render_batch) and the
-// autotuner (ptx_autotune).
+// (ptx_render), the batched path (ptx_render_batch) and the autotuner
+// (ptx_autotune).
 //
-// The batched path keeps one set of path queues, hit records, per-path
-// results and a stack spill area on the device (the RenderEngine) and reuses
-// them across calls.  Work is organised in rounds: a round holds one chunk
-// (samples [s0, s0 + k) of every pixel) from each job that still has samples,
-// sized so the round fits the path capacity.  A round is: shade every slot at
-// its cached primary hit, then one bounce per replay of a captured hipGraph
-// (prologue, extend, shade), then add the round's samples to the films in
-// sample order.  The host never waits between bounces or rounds.
+// The batched path keeps per-path results, a stack spill area and (for the
+// wavefront form) path queues and hit records on the device (the
+// RenderEngine) and reuses them across calls.  Work is organised in rounds: a
+// round holds one chunk (samples [s0, s0 + k) of every pixel) from each job
+// that still has samples, sized so the round fits the path capacity.  A round
+// is either one persistent path-regeneration launch (k_paths), or, in the
+// wavefront form, shading every slot at its cached primary hit and then one
+// bounce per replay of a captured hipGraph (prologue, extend, shade); either
+// way the round ends by adding its samples to the films in sample order.  The
+// host never waits between bounces or rounds.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -94,8 +96,9 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       ext_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(0));
       shade_grid = (uint32_t)(dev->num_cus * 8);
       mega_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(2));
+      paths_grid = (uint32_t)(dev->num_cus * paths_blocks_per_cu());
    }
-   const uint32_t stride_threads = std::max(ext_grid, mega_grid) * 256;
+   const uint32_t stride_threads = std::max({ext_grid, mega_grid, paths_grid}) * 256;
    const size_t need_spill = max_stack > lds ? (size_t)(max_stack - lds) * stride_threads : 0;
    bool graphs_stale = false;
    int rc;
@@ -207,9 +210,16 @@ int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
    return PTX_OK;
 }
 
-void RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth) {
+void RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes) {
    uint32_t nslots = 0;
    for (const Chunk &c : r) nslots = std::max(nslots, c.base + c.k * c.npix);
+   const uint32_t mode = dev->tuning.render_mode;
+   if (mode != PTX_RENDER_WAVEFRONT && r.size() <= paths_max_chunks() && nscenes <= paths_max_scenes()) {
+      launch_paths(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, nscenes, res, words + kWork, spill, stack_cap,
+                   std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
+      launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
+      return;
+   }
    launch_round_start(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, res, qa, words + kCountA,
                       words + kBounce, std::min<uint32_t>(shade_grid, (nslots + 255) / 256), stream);
    for (uint32_t b = 1; b <= depth; b++) (void)hipGraphLaunch(b & 1 ? g_ab : g_ba, stream);
@@ -276,7 +286,7 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
    for (const auto &r : rounds) {
       uint32_t depth = 0;
       for (const Chunk &c : r) depth = std::max(depth, jobs[c.job].desc.max_depth);
-      round(d_chunks + off, r, depth);
+      round(d_chunks + off, r, depth, (uint32_t)scenes.size());
       off += r.size();
    }
    for (uint32_t j = 0; j < nj; j++) {
@@ -447,7 +457,7 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
    (void)hipFree(d_rays);
    (void)hipFree(d_hits);
    dev->tuning = best;
-   // 2. path capacity of the wavefront queues, on a probe render of the scene
+   // 2. round form and path slots per round, on a probe render of the scene
    if (rc == PTX_OK) {
       ptx_job_desc job{};
       const float c[3] = {(bb[0] + bb[3]) / 2, (bb[1] + bb[4]) / 2, (bb[2] + bb[5]) / 2};
@@ -469,22 +479,26 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
       for (auto &r : rj) { r.desc = job; r.scene_index = 0; r.npix = 256 * 256; }
       std::vector<float *> films(4, film.data());
       double best_s = 1e30;
-      uint32_t best_paths = best.paths_per_wave;
+      uint32_t best_paths = best.paths_per_wave, best_mode = best.render_mode;
       std::lock_guard<std::mutex> lock(dev->mu);
-      for (uint32_t paths : {1u << 20, 1u << 21, 1u << 22, 1u << 23}) {
-         dev->tuning.paths_per_wave = paths;
-         double t_min = 1e30;
-         for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
-            const double t0 = now_s();
-            rc = engine_of(dev)->run_batch(rj, {scene}, films);
-            t_min = std::min(t_min, now_s() - t0);
+      for (uint32_t mode : {(uint32_t)PTX_RENDER_REGEN, (uint32_t)PTX_RENDER_WAVEFRONT})
+         for (uint32_t paths : {1u << 20, 1u << 21, 1u << 22, 1u << 23}) {
+            dev->tuning.paths_per_wave = paths;
+            dev->tuning.render_mode = mode;
+            double t_min = 1e30;
+            for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
+               const double t0 = now_s();
+               rc = engine_of(dev)->run_batch(rj, {scene}, films);
+               t_min = std::min(t_min, now_s() - t0);
+            }
+            if (rc == PTX_OK && t_min < best_s) {
+               best_s = t_min;
+               best_paths = paths;
+               best_mode = mode;
+            }
          }
-         if (rc == PTX_OK && t_min < best_s) {
-            best_s = t_min;
-            best_paths = paths;
-         }
-      }
       best.paths_per_wave = best_paths;
+      best.render_mode = best_mode;
    }
    (void)hipEventDestroy(e0);
    (void)hipEventDestroy(e1);

@@ -41,7 +41,7 @@ struct RenderEngine {
    uint32_t scenes_cap = 0;
    Chunk *d_chunks = nullptr;
    uint32_t chunks_cap = 0;
-   uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0;
+   uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0, paths_grid = 0;
    hipGraphExec_t g_ab = nullptr, g_ba = nullptr;
 
    ~RenderEngine();
@@ -52,8 +52,8 @@ struct RenderEngine {
    // primary hits.  J.film / J.primary are set.
    int start_job(JobDev &J, const DevScene &S, uint32_t npix);
    // One round over the chunks in d_round (a device copy of r); the job and
-   // scene tables must hold the round's jobs.
-   void round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth);
+   // scene tables must hold the round's jobs, the scene table nscenes entries.
+   void round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes);
    int finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host);
    void free_job(JobDev &J);
    uint32_t capacity_for(uint64_t total_paths, uint32_t max_npix) const;

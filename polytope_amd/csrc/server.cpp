@@ -185,7 +185,7 @@ void ptx_server::loop() {
          if (!chunks.empty()) {
             (void)hipMemcpyAsync(eng.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice,
                                  stream);
-            eng.round(eng.d_chunks, chunks, depth);
+            eng.round(eng.d_chunks, chunks, depth, kMaxServerScenes);
          }
          const hipError_t e = hipStreamSynchronize(stream);
          std::lock_guard<std::mutex> lock(mu);

@@ -186,9 +186,11 @@ struct Frame {
    float lx, ly, lz, nx, ny, nz, t1x, t1y, t1z, t2x, t2y, t2z, ex, ey, ez;
 };
 
-__device__ __forceinline__ void hit_frame(const float *__restrict__ v, const float *__restrict__ cn,
-                                          const Ray &r, float u, float vv, float w, uint32_t repeats,
-                                          Frame &f) {
+// use_cn selects the interpolated corner normals cn (which must be readable
+// either way) over the geometric normal.
+__device__ __forceinline__ void hit_frame_sel(const float *__restrict__ v, const float *__restrict__ cn, bool use_cn,
+                                              const Ray &r, float u, float vv, float w, uint32_t repeats,
+                                              Frame &f) {
    const float v0x = v[0], v0y = v[1], v0z = v[2];
    const float v1x = v[3], v1y = v[4], v1z = v[5];
    const float v2x = v[6], v2y = v[7], v2z = v[8];
@@ -203,7 +205,7 @@ __device__ __forceinline__ void hit_frame(const float *__restrict__ v, const flo
    const float e1x = v2x - v1x, e1y = v2y - v1y, e1z = v2z - v1z;
    const float e2x = v0x - v2x, e2y = v0y - v2y, e2z = v0z - v2z;
    float nx, ny, nz;
-   if (cn) {
+   if (use_cn) {
       nx = cn[0] * u + cn[3] * vv + cn[6] * w;
       ny = cn[1] * u + cn[4] * vv + cn[7] * w;
       nz = cn[2] * u + cn[5] * vv + cn[8] * w;
@@ -247,6 +249,12 @@ __device__ __forceinline__ void hit_frame(const float *__restrict__ v, const flo
       const float ld = 1.0f / __builtin_sqrtf(sx * sx + sy * sy + sz * sz);
       f.t2x = sx * ld; f.t2y = sy * ld; f.t2z = sz * ld;
    }
+}
+
+__device__ __forceinline__ void hit_frame(const float *__restrict__ v, const float *__restrict__ cn,
+                                          const Ray &r, float u, float vv, float w, uint32_t repeats,
+                                          Frame &f) {
+   hit_frame_sel(v, cn ? cn : v, cn != nullptr, r, u, vv, w, repeats, f);
 }
 
 // ---------------------------------------------------------------- traversal

@@ -451,6 +451,69 @@ def batched_render_workload(subdiv: int = 8, instances: int = 32, jobs: int = 16
     return meshes, (150.0, 170.0, 200.0), job_list
 
 
+def stress_workload(n_rays: int = 1 << 24, spheres: int = 32, n_lat: int = 256, n_lon: int = 512,
+                    grid_n: int = 1024, seed: int = 5):
+    """Stress and watertightness workload: ``spheres`` displaced lat-long spheres of
+    2*n_lat*n_lon faces (262,144 at the defaults; radius 1, +/-5 % radial value
+    noise) on an 8-wide layout above a grid_n x grid_n sliver grid (2*grid_n^2
+    faces, cells up to 1000:1), every mesh in world space with an identity
+    transform so feature positions are exact floats.  Meshes alternate Lambert and
+    mirror.  Of the n_rays rays the first quarter is aimed exactly at shared
+    features (vertices and float32 edge midpoints; half on the spheres, from
+    origins 2.5 radii around the sphere's centre, half on the grid, from origins
+    at distance 4 in random directions); the rest are incoherent (origin uniform in
+    the scene box, direction uniform).  Returns (meshes, rays, n_feature, t_target)
+    with t_target[i] = |target - origin| for the feature rays."""
+    rng = Rng(seed)
+    sv, sf = latlong_sphere(n_lat, n_lon, 1.0)
+    unit = sv.astype(np.float64)
+    meshes = []
+    centers = []
+    cols = 8
+    for i in range(spheres):
+        cx = (i % cols - (cols - 1) / 2.0) * 3.0
+        cz = (i // cols - (max(1, (spheres + cols - 1) // cols) - 1) / 2.0) * 3.0
+        c = np.array([cx, 1.2, cz])
+        d = unit * (1.0 + 0.05 * value_noise(unit, seed + i, 3.0))[:, None] + c
+        centers.append(c)
+        u = rng.uniform(3)
+        meshes.append(MeshData(d.astype(np.float32), sf, brdf=MIRROR if i % 3 == 2 else LAMBERT,
+                               refl=tuple(float(0.2 + 0.6 * x) for x in u)))
+    size = 3.0 * cols + 6.0
+    gv, gf = sliver_grid(grid_n, size, 1000.0, 0.0, seed)
+    meshes.append(MeshData(gv, gf, refl=(0.5, 0.5, 0.5)))
+    n_feat = n_rays // 4
+    n_sph = n_feat // 2
+    n_grid = n_feat - n_sph
+
+    def targets(verts, faces, n):
+        fi = (rng.uniform(n) * faces.shape[0]).astype(np.int64) % faces.shape[0]
+        ci = (rng.uniform(n) * 3).astype(np.int64) % 3
+        a = verts[faces[fi, ci]].astype(np.float32)
+        b = verts[faces[fi, (ci + 1) % 3]].astype(np.float32)
+        mid = ((a + b) * np.float32(0.5)).astype(np.float32)
+        return np.where((np.arange(n) % 2 == 0)[:, None], a, mid)
+
+    which = (rng.uniform(n_sph) * spheres).astype(np.int64) % spheres
+    tgt_s = np.zeros((n_sph, 3), np.float32)
+    org_s = np.zeros((n_sph, 3), np.float64)
+    for i in range(spheres):
+        sel = np.nonzero(which == i)[0]
+        tgt_s[sel] = targets(meshes[i].verts, sf, len(sel))
+        org_s[sel] = centers[i] + unit_vectors(len(sel), rng) * 2.5
+    tgt_g = targets(gv, gf, n_grid)
+    org_g = tgt_g.astype(np.float64) + unit_vectors(n_grid, rng) * 4.0
+    tgt = np.concatenate([tgt_s, tgt_g], 0).astype(np.float64)
+    org = np.concatenate([org_s, org_g], 0).astype(np.float32).astype(np.float64)
+    d = tgt - org
+    t_target = np.linalg.norm(d, axis=1)
+    d /= t_target[:, None]
+    feat = np.concatenate([org, d], 1).astype(np.float32)
+    lo, hi = scene_bounds(meshes)
+    inc = rays_incoherent(lo, hi, n_rays - n_feat, seed + 100)
+    return meshes, np.concatenate([feat, inc], 0), n_feat, t_target.astype(np.float32)
+
+
 def serving_requests(n: int, rate_hz: float, seed: int = 4, subdivs=(5, 6, 7, 8)):
     """Poisson request stream for the render service: ``n`` jobs at rate ``rate_hz``
     over displaced-icosphere scenes (subdivisions ``subdivs``: 20k to 1.3M faces),

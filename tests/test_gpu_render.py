@@ -1,5 +1,6 @@
-"""GPU parity of the render paths (ptx_render: plain, ptx_render_batch:
-batched wavefront) against the reference's golden film and the oracle.
+"""GPU parity of the render paths (ptx_render: plain; ptx_render_batch: path
+regeneration and wavefront rounds) against the reference's golden film and the
+oracle.
 
 Bar: per-pixel relative error <= 1e-4 against the oracle under the same RNG
 stream (the paths share the oracle's operation order, so the films are checked
@@ -13,6 +14,7 @@ import pytest
 
 import oracle as O
 from polytope_amd import Device, Scene, render, render_batch
+from polytope_amd import _abi
 from polytope_amd import scenes as S
 from tests import _fixtures as F
 
@@ -35,6 +37,17 @@ def rel_err(a, b):
     return float((np.abs(a - b) / np.maximum(np.abs(b), 1e-3)).max())
 
 
+MODES = [_abi.RENDER_REGEN, _abi.RENDER_WAVEFRONT]
+
+
+def batch(dev, jobs, mode, paths_per_wave=0):
+    dev.set_tuning(paths_per_wave=paths_per_wave, render_mode=mode)
+    try:
+        return render_batch(jobs)
+    finally:
+        dev.set_tuning()
+
+
 def test_render_mirror_matches_reference_golden(dev):
     d = F.load("mirror_render")
     cam, spp, depth = F.camera_of(d)
@@ -42,8 +55,9 @@ def test_render_mirror_matches_reference_golden(dev):
     sc = Scene(dev, F.meshes_of(d), d["sky"])
     plain = render(sc, cam, spp, depth, seed=1)
     assert np.array_equal(plain.view(np.uint32), d["ref_film"].view(np.uint32))
-    (batched,) = render_batch([(sc, cam, spp, depth, 1)])
-    assert np.array_equal(batched.view(np.uint32), d["ref_film"].view(np.uint32))
+    for mode in MODES:
+        (batched,) = batch(dev, [(sc, cam, spp, depth, 1)], mode)
+        assert np.array_equal(batched.view(np.uint32), d["ref_film"].view(np.uint32)), mode
 
 
 def _lambert_scene(w=80, h=60):
@@ -58,11 +72,12 @@ def test_render_plain_and_batched_match_oracle(dev, spp, depth):
     ref = O.OracleScene(meshes, sky).render(cam, spp, depth, seed=5)
     sc = Scene(dev, meshes, sky)
     plain = render(sc, cam, spp, depth, seed=5)
-    (batched,) = render_batch([(sc, cam, spp, depth, 5)])
     assert rel_err(plain, ref) <= REL_TOL
-    assert rel_err(batched, ref) <= REL_TOL
     assert np.array_equal(plain.view(np.uint32), ref.view(np.uint32))
-    assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
+    for mode in MODES:
+        (batched,) = batch(dev, [(sc, cam, spp, depth, 5)], mode)
+        assert rel_err(batched, ref) <= REL_TOL, mode
+        assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32)), mode
 
 
 def test_render_mixed_materials_emitter_and_debug_pixel(dev):
@@ -78,12 +93,14 @@ def test_render_mixed_materials_emitter_and_debug_pixel(dev):
     ref = O.OracleScene(meshes, sky).render(cam, 4, 6, seed=9)
     sc = Scene(dev, meshes, sky)
     plain = render(sc, cam, 4, 6, seed=9)
-    (batched,) = render_batch([(sc, cam, 4, 6, 9)])
     assert np.array_equal(plain.view(np.uint32), ref.view(np.uint32))
-    assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
+    for mode in MODES:
+        (batched,) = batch(dev, [(sc, cam, 4, 6, 9)], mode)
+        assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32)), mode
 
 
-def test_render_batch_alone_equals_mixed(dev):
+@pytest.mark.parametrize("mode", MODES)
+def test_render_batch_alone_equals_mixed(dev, mode):
     # jobs of uneven resolution, spp and depth over two scenes: each film is the same
     # rendered alone, in the mix, and with a path capacity small enough to force many rounds
     meshes_a, cam_a, sky_a = _lambert_scene(64, 48)
@@ -95,13 +112,10 @@ def test_render_batch_alone_equals_mixed(dev):
     cam_b = S.Camera(S.look_at((0.3, 0.5, -3.0), (0.0, 0.0, 0.0)), 50.0, 40, 30)
     cam_c = S.Camera(S.look_at((0.3, 0.5, -3.0), (0.0, 0.0, 0.0)), 70.0, 96, 40)
     jobs = [(sa, cam_a, 5, 4, 11), (sb, cam_b, 9, 7, 12), (sb, cam_c, 2, 3, 13), (sa, cam_a, 1, 2, 14)]
-    alone = [render_batch([j])[0] for j in jobs]
-    mixed = render_batch(jobs)
-    dev.set_tuning(paths_per_wave=5000)
-    try:
-        small = render_batch(jobs)
-    finally:
-        dev.set_tuning()
+    print(f"case alone/mixed mode={mode}", flush=True)
+    alone = [batch(dev, [j], mode)[0] for j in jobs]
+    mixed = batch(dev, jobs, mode)
+    small = batch(dev, jobs, mode, paths_per_wave=5000)
     for a, m, s, j in zip(alone, mixed, small, jobs):
         assert np.array_equal(a.view(np.uint32), m.view(np.uint32))
         assert np.array_equal(a.view(np.uint32), s.view(np.uint32))
@@ -109,3 +123,16 @@ def test_render_batch_alone_equals_mixed(dev):
         assert np.array_equal(a.view(np.uint32), plain.view(np.uint32))
     ref = O.OracleScene(meshes_b, (100.0, 120.0, 150.0)).render(cam_b, 9, 7, seed=12)
     assert np.array_equal(mixed[1].view(np.uint32), ref.view(np.uint32))
+
+
+def test_render_batch_more_chunks_than_lds_table(dev):
+    # 70 jobs in one round: more chunks than the regeneration kernel's LDS table,
+    # so the default mode runs that round in wavefront form; films are unchanged
+    meshes, cam, sky = _lambert_scene(16, 12)
+    sc = Scene(dev, meshes, sky)
+    jobs = [(sc, cam, 1 + i % 3, 1 + i % 4, 100 + i) for i in range(70)]
+    print("case 70 jobs", flush=True)
+    mixed = render_batch(jobs)
+    for i in (0, 1, 33, 69):
+        (alone,) = render_batch([jobs[i]])
+        assert np.array_equal(alone.view(np.uint32), mixed[i].view(np.uint32))

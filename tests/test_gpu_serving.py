@@ -35,13 +35,16 @@ def _scenes(dev):
     return Scene(dev, meshes_a, sky_a), cam_a, Scene(dev, meshes_b, (100.0, 120.0, 150.0)), meshes_b
 
 
-def test_server_results_equal_alone(dev):
+@pytest.mark.parametrize("mode", [_abi.RENDER_REGEN, _abi.RENDER_WAVEFRONT])
+def test_server_results_equal_alone(dev, mode):
     sa, cam_a, sb, _ = _scenes(dev)
+    print(f"case server alone mode={mode}", flush=True)
     cam_b = S.Camera(S.look_at((0.3, 0.5, -3.0), (0.0, 0.0, 0.0)), 50.0, 40, 30)
     cam_c = S.Camera(S.look_at((0.3, 0.5, -3.0), (0.0, 0.0, 0.0)), 70.0, 96, 40)
     jobs = [(sa, cam_a, 6, 4, 21), (sb, cam_b, 9, 7, 22), (sb, cam_c, 3, 3, 23), (sa, cam_a, 2, 2, 24),
             (sb, cam_b, 17, 5, 25)]
     alone = [render_batch([j])[0] for j in jobs]
+    dev.set_tuning(render_mode=mode)
     srv = Server(dev, hbm_budget_bytes=8 << 30, paths_per_wave=20000)
     try:
         ids = []
@@ -57,6 +60,7 @@ def test_server_results_equal_alone(dev):
             assert np.array_equal(film.view(np.uint32), a.view(np.uint32))
     finally:
         srv.close()
+        dev.set_tuning()
 
 
 def test_server_budget_admission_and_rejection(dev):
