@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--render-subdiv", type=int, default=8)
     ap.add_argument("--render-instances", type=int, default=32)
     ap.add_argument("--render-spp", type=int, default=64)
+    ap.add_argument("--shade-lanes", type=int, default=0,
+                    help="lanes a path-regeneration wave gathers before it shades (0 = default)")
+    ap.add_argument("--render-paths", type=int, default=0, help="path slots per round (0 = default)")
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-rate", type=float, default=8.0)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -108,12 +111,13 @@ def render_section(dev, args):
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     samples = sum(c.width * c.height * spp for c, spp, _, _ in jobs)
     batch = [(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
+    dev.set_tuning(shade_lanes=args.shade_lanes, paths_per_wave=args.render_paths)
     render_batch([(scene, jobs[0][0], 1, 8, 1)])                  # warm the engine and graphs
     tb = time.perf_counter()
     films_b = render_batch(batch)
     t_batched = time.perf_counter() - tb
     print(f"[bench] batched: {samples / t_batched / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
-    dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT)
+    dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT, paths_per_wave=args.render_paths)
     render_batch([(scene, jobs[0][0], 1, 8, 1)])
     tw = time.perf_counter()
     films_w = render_batch(batch)

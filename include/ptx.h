@@ -186,6 +186,8 @@ typedef struct ptx_tuning {
    uint32_t blocks_per_cu;   /* persistent blocks per CU; 0 = occupancy limit         */
    uint32_t render_mode;     /* PTX_RENDER_*: AUTO = regeneration where a round fits  */
                              /* its LDS tables (64 chunks, 64 scenes), else wavefront */
+   uint32_t shade_lanes;     /* regeneration: lanes of a wave that must wait for a    */
+                             /* shading step before the wave shades (1..64; 0 = 16)  */
 } ptx_tuning;
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);

@@ -18,6 +18,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
 REF_DRIVER = HERE / "_ref" / "ref_driver"
+# the reference's runner API driving the GPU library through poly::GpuRunner
+GPU_RUNNER_CHECK = HERE / "_ref" / "gpu_runner_check"
 REFERENCE_ROOT = Path("/root/reference")
 
 HIT_RECORD = np.dtype([("hit", "<u4"), ("mesh", "<u4"), ("face", "<u4"), ("t", "<f4"), ("u", "<f4"),

@@ -265,6 +265,7 @@ int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
    if (t->lds_stack && t->lds_stack != 8 && t->lds_stack != 12 && t->lds_stack != 16 && t->lds_stack != 24)
       return set_error(PTX_ERR_INVALID, "lds_stack must be 0, 8, 12, 16 or 24");
    if (t->render_mode > PTX_RENDER_WAVEFRONT) return set_error(PTX_ERR_INVALID, "render_mode must be 0, 1 or 2");
+   if (t->shade_lanes > 64) return set_error(PTX_ERR_INVALID, "shade_lanes must be 0..64");
    dev->tuning = *t;
    return PTX_OK;
 }
@@ -275,6 +276,7 @@ int ptx_get_tuning(ptx_device *dev, ptx_tuning *t) {
    if (!t->lds_stack) t->lds_stack = lds_depth_of(dev);
    if (!t->block_size) t->block_size = 256;
    if (!t->refill_lanes) t->refill_lanes = 1;
+   if (!t->shade_lanes) t->shade_lanes = RenderEngine::kDefaultShadeLanes;
    return PTX_OK;
 }
 

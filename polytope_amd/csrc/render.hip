@@ -586,10 +586,20 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
 // step issues one mode's loads for every lane before any compute (the traversal
 // step's node or faces, the primary record, the hit's mesh index, its mesh
 // record, its faces), as in the closest-hit loop.  The round's chunk table and
-// scene table sit in LDS.
-enum : uint32_t { kPIdle = 0, kPPrim = 1, kPTrace = 2, kPShade1 = 3, kPShade2 = 4, kPShade3 = 5 };
-constexpr uint32_t kPModeMask = 7u, kPDbg = 8u;
+// scene table sit in LDS.  The shading steps (primary bounce, hit frame and
+// BRDF sample) cost several traversal steps and run for the whole wave when any
+// lane takes them, so a wave shades only when at least shade_min of its lanes
+// wait for it, or when none is tracing; until then those lanes hold.
+enum : uint32_t { kPIdle = 0, kPPrim = 1, kPTrace = 2, kPShade1 = 3, kPShade2 = 4, kPShade3 = 5, kPHold = 6 };
+// Lane word: bits 0-2 state, bit 3 the reference's debug pixel, bits 4-9 the
+// path's chunk in the round, bits 10-31 its bounce index.
+constexpr uint32_t kPModeMask = 7u, kPDbg = 8u, kPChunkShift = 4, kPBounceShift = 10;
 constexpr uint32_t kMaxRoundScenes = 64;
+static_assert(kMaxRoundChunks <= 64, "chunk index has 6 bits in the lane word");
+
+__device__ __forceinline__ uint32_t lw_chunk(uint32_t w) { return (w >> kPChunkShift) & 63u; }
+__device__ __forceinline__ uint32_t lw_bounce(uint32_t w) { return w >> kPBounceShift; }
+__device__ __forceinline__ uint32_t lw_with_state(uint32_t w, uint32_t state) { return (w & ~kPModeMask) | state; }
 
 struct LdsChunk {
    uint32_t base, npix, s0, width;
@@ -607,7 +617,7 @@ __device__ __forceinline__ uint32_t lds_find_chunk(const LdsChunk *c, uint32_t n
    return lo;
 }
 
-// Starts the traversal of ray s.r in scene S; returns the next mode, or kPIdle
+// Starts the traversal of ray s.r in scene S; returns the next state, or kPIdle
 // after writing the miss result when the root box is missed.
 __device__ __forceinline__ uint32_t p_root(const DevScene &S, Trav &s, float *res, uint32_t slot,
                                            const float src[3]) {
@@ -618,9 +628,9 @@ __device__ __forceinline__ uint32_t p_root(const DevScene &S, Trav &s, float *re
 }
 
 template <int kLds>
-__global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots,
-                                               const JobDev *jobs, const DevScene *scenes, uint32_t nscenes,
-                                               float *res, uint32_t *work, uint4 *spill, uint32_t max_stack) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
+        uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min) {
    __shared__ uint32_t lds_ref[kLds * 256];
    __shared__ float lds_near[kLds * 256];
    __shared__ uint16_t lds_cnt[kLds * 256];
@@ -648,8 +658,8 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
    s.cur = kNone;
    s.cnt = 0;
    s.sp = 0;
-   uint32_t mode = kPIdle;       // state | kPDbg (the reference's debug pixel)
-   uint32_t slot = 0, ci = 0, b = 0;
+   uint32_t mode = kPIdle;       // lane word
+   uint32_t slot = 0;
    uint32_t t0 = 0;              // Shade1 -> Shade2: mesh; Shade2 -> Shade3: brdf
    float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2 -> Shade3: reflectance
    float src[3] = {1.f, 1.f, 1.f};
@@ -673,15 +683,20 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
             srcq.pool_left -= take;
          }
       }
-      const uint32_t m = mode & kPModeMask;
-      if (!__any(m != kPIdle)) break;
+      const uint32_t m0 = mode & kPModeMask;
+      if (!__any(m0 != kPIdle)) break;
+      const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3;
+      const bool run_heavy =
+            (uint32_t)__popcll(__ballot(heavy)) >= shade_min || !__any(m0 == kPTrace || m0 == kPShade1);
+      const uint32_t m = heavy && !run_heavy ? kPHold : m0;
       // ------------------------------------------------------------ fetch
       Fetched f;
       if (m == kPTrace) {
          fetch(s, nodes, tris, f);
       } else if (m == kPPrim) {
-         ci = lds_find_chunk(sch, nchunks, slot);
-         const LdsChunk &c = sch[ci];
+         const uint32_t cf = lds_find_chunk(sch, nchunks, slot);
+         mode = kPPrim | (cf << kPChunkShift);
+         const LdsChunk &c = sch[cf];
          const uint32_t pixel = (slot - c.base) % c.npix;
          const float4 *pr = reinterpret_cast<const float4 *>(c.primary + 16 * (size_t)pixel);
          const float4 q0 = pr[0], q1 = pr[1], q2 = pr[2], q3 = pr[3];
@@ -690,9 +705,9 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
          f.q[8] = q2.x; f.q[9] = q2.y; f.q[10] = q2.z; f.q[11] = q2.w;
          f.q[12] = q3.x; f.q[13] = q3.y; f.q[14] = q3.z; f.q[15] = q3.w;
       } else if (m == kPShade1) {
-         f.q[0] = __uint_as_float(ssc[sch[ci].scene].pairs[2 * (size_t)s.hit]);
+         f.q[0] = __uint_as_float(ssc[sch[lw_chunk(mode)].scene].pairs[2 * (size_t)s.hit]);
       } else if (m == kPShade2) {
-         const uint32_t *mr = reinterpret_cast<const uint32_t *>(ssc[sch[ci].scene].meshes + t0);
+         const uint32_t *mr = reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0);
 #pragma unroll
          for (int k = 0; k < 9; k++) f.q[k] = __uint_as_float(mr[k]);
          const float *v = tris + 9 * (size_t)s.hit;
@@ -700,7 +715,7 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
          for (int k = 0; k < 9; k++) f.q[9 + k] = v[k];
       } else if (m == kPShade3) {
          const float *v = tris + 9 * (size_t)s.hit;
-         const float *nrm = ssc[sch[ci].scene].normals + 9 * (size_t)s.hit;
+         const float *nrm = ssc[sch[lw_chunk(mode)].scene].normals + 9 * (size_t)s.hit;
 #pragma unroll
          for (int k = 0; k < 9; k++) { f.q[k] = v[k]; f.q[9 + k] = nrm[k]; }
       }
@@ -708,14 +723,14 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
       if (m == kPTrace) {
          if (!advance<kLds>(s, f, st, (mode & kPDbg) != 0)) {
             if (s.hit == kNone) {
-               write_res(res, slot, miss_radiance(ssc[sch[ci].scene], src));
+               write_res(res, slot, miss_radiance(ssc[sch[lw_chunk(mode)].scene], src));
                mode = kPIdle;
             } else {
-               mode = (mode & kPDbg) | kPShade1;
+               mode = lw_with_state(mode, kPShade1);
             }
          }
       } else if (m == kPPrim) {
-         const LdsChunk &c = sch[ci];
+         const LdsChunk &c = sch[lw_chunk(mode)];
          const uint32_t local = slot - c.base;
          const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
          const uint32_t w0 = __float_as_uint(f.q[0]);
@@ -743,16 +758,16 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
                make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
                nodes = S.nodes;
                tris = S.tris;
-               b = 1;
-               mode = dbg | p_root(S, s, res, slot, src);
-               if ((mode & kPModeMask) == kPIdle) mode = kPIdle;
+               const uint32_t next = p_root(S, s, res, slot, src);
+               mode = next == kPIdle ? kPIdle : (mode & ~kPModeMask) | dbg | (1u << kPBounceShift) | next;
             }
          }
       } else if (m == kPShade1) {
          t0 = __float_as_uint(f.q[0]);
-         mode = (mode & kPDbg) | kPShade2;
+         mode = lw_with_state(mode, kPShade2);
       } else if (m == kPShade2 || m == kPShade3) {
-         const LdsChunk &c = sch[ci];
+         const LdsChunk &c = sch[lw_chunk(mode)];
+         const uint32_t b = lw_bounce(mode);
          bool go = true;
          if (m == kPShade2) {
             const int32_t brdf = __float_as_int(f.q[0]);
@@ -770,7 +785,7 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
             } else if (has_normals && ssc[c.scene].normals) {
                t0 = (uint32_t)brdf;
                t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
-               mode = (mode & kPDbg) | kPShade3;
+               mode = lw_with_state(mode, kPShade3);
                go = false;
             } else {
                t0 = (uint32_t)brdf;
@@ -796,10 +811,8 @@ __global__ void __launch_bounds__(256) k_paths(const Chunk *chunks, uint32_t nch
                mode = kPIdle;
             } else {
                make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
-               b++;
-               const uint32_t dbg = mode & kPDbg;
-               mode = dbg | p_root(ssc[c.scene], s, res, slot, src);
-               if ((mode & kPModeMask) == kPIdle) mode = kPIdle;
+               const uint32_t next = p_root(ssc[c.scene], s, res, slot, src);
+               mode = next == kPIdle ? kPIdle : ((mode & ~kPModeMask) + (1u << kPBounceShift)) | next;
             }
          }
       }
@@ -875,11 +888,11 @@ uint32_t paths_max_chunks() { return kMaxRoundChunks; }
 uint32_t paths_max_scenes() { return dev::kMaxRoundScenes; }
 
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
-                  uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t grid,
-                  hipStream_t st) {
+                  uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
+                  uint32_t grid, hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
    hipLaunchKernelGGL(dev::k_paths<kRenderLds>, dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, scenes,
-                      nscenes, res, work, spill, max_stack);
+                      nscenes, res, work, spill, max_stack, shade_min);
 }
 
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {

@@ -82,12 +82,18 @@ void RenderEngine::release() {
    g_ab = g_ba = nullptr;
    dfree(qa); dfree(qb); dfree(hits); dfree(res); dfree(words); dfree(spill);
    dfree(d_jobs); dfree(d_scenes); dfree(d_chunks);
-   cap = 0;
+   cap = qcap = 0;
    spill_entries = 0;
+   graphs_stale = true;
 }
 
-size_t RenderEngine::bytes_for(uint32_t paths) {
-   return (size_t)paths * (2 * 16 * sizeof(float) + 8 * sizeof(float) + 3 * sizeof(float));
+size_t RenderEngine::bytes_for(uint32_t paths, bool queues) {
+   return (size_t)paths * (3 * sizeof(float) + (queues ? 2 * 16 * sizeof(float) + 8 * sizeof(float) : 0));
+}
+
+bool RenderEngine::regen_round(uint32_t nchunks, uint32_t nscenes) const {
+   return dev->tuning.render_mode != PTX_RENDER_WAVEFRONT && nchunks <= paths_max_chunks() &&
+          nscenes <= paths_max_scenes();
 }
 
 int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uint32_t nscenes, uint32_t nchunks) {
@@ -100,12 +106,10 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
    }
    const uint32_t stride_threads = std::max({ext_grid, mega_grid, paths_grid}) * 256;
    const size_t need_spill = max_stack > lds ? (size_t)(max_stack - lds) * stride_threads : 0;
-   bool graphs_stale = false;
    int rc;
    if (paths > cap) {
-      dfree(qa); dfree(qb); dfree(hits); dfree(res);
-      if ((rc = dalloc(&qa, (size_t)paths * 16)) || (rc = dalloc(&qb, (size_t)paths * 16)) ||
-          (rc = dalloc(&hits, (size_t)paths * 8)) || (rc = dalloc(&res, (size_t)paths * 3))) {
+      dfree(res);
+      if ((rc = dalloc(&res, (size_t)paths * 3))) {
          release();
          return rc;
       }
@@ -143,6 +147,22 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       chunks_cap = std::max(nchunks, 1024u);
       if ((rc = dalloc(&d_chunks, chunks_cap))) return rc;
    }
+   return PTX_OK;
+}
+
+int RenderEngine::ensure_queues(uint32_t need) {
+   int rc;
+   if (qcap < need) {
+      dfree(qa); dfree(qb); dfree(hits);
+      qcap = 0;
+      if ((rc = dalloc(&qa, (size_t)need * 16)) || (rc = dalloc(&qb, (size_t)need * 16)) ||
+          (rc = dalloc(&hits, (size_t)need * 8))) {
+         dfree(qa); dfree(qb); dfree(hits);
+         return rc;
+      }
+      qcap = need;
+      graphs_stale = true;
+   }
    if (graphs_stale || !g_ab) {
       if (g_ab) (void)hipGraphExecDestroy(g_ab);
       if (g_ba) (void)hipGraphExecDestroy(g_ba);
@@ -161,18 +181,20 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
          if (ie != hipSuccess) return set_error(PTX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
          (dir == 0 ? g_ab : g_ba) = ex;
       }
+      graphs_stale = false;
    }
    return PTX_OK;
 }
 
-// Rounds: each holds at most one chunk per job; chunk sizes share the path
-// capacity between the jobs that still have samples.
-std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jobs, uint32_t cap) {
+// Rounds: each holds at most one chunk per job and at most max_chunks chunks;
+// chunk sizes share the path capacity between the jobs that still have samples.
+std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jobs, uint32_t cap,
+                                                uint32_t max_chunks) {
    std::vector<uint32_t> next(jobs.size(), 0);
    std::vector<std::vector<Chunk>> rounds;
    while (true) {
       std::vector<uint32_t> active;
-      for (uint32_t j = 0; j < jobs.size(); j++)
+      for (uint32_t j = 0; j < jobs.size() && active.size() < max_chunks; j++)
          if (next[j] < jobs[j].desc.spp) active.push_back(j);
       if (active.empty()) break;
       std::vector<Chunk> round;
@@ -210,20 +232,23 @@ int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
    return PTX_OK;
 }
 
-void RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes) {
+int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes) {
    uint32_t nslots = 0;
    for (const Chunk &c : r) nslots = std::max(nslots, c.base + c.k * c.npix);
-   const uint32_t mode = dev->tuning.render_mode;
-   if (mode != PTX_RENDER_WAVEFRONT && r.size() <= paths_max_chunks() && nscenes <= paths_max_scenes()) {
+   if (regen_round((uint32_t)r.size(), nscenes)) {
+      const uint32_t shade_min = dev->tuning.shade_lanes ? dev->tuning.shade_lanes : kDefaultShadeLanes;
       launch_paths(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, nscenes, res, words + kWork, spill, stack_cap,
-                   std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
+                   shade_min, std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
       launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
-      return;
+      return PTX_OK;
    }
+   const int rc = ensure_queues(nslots);
+   if (rc) return rc;
    launch_round_start(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, res, qa, words + kCountA,
                       words + kBounce, std::min<uint32_t>(shade_grid, (nslots + 255) / 256), stream);
    for (uint32_t b = 1; b <= depth; b++) (void)hipGraphLaunch(b & 1 ? g_ab : g_ba, stream);
    launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
+   return PTX_OK;
 }
 
 int RenderEngine::finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host) {
@@ -238,10 +263,11 @@ void RenderEngine::free_job(JobDev &J) {
    dfree(J.primary);
 }
 
-uint32_t RenderEngine::capacity_for(uint64_t total_paths, uint32_t max_npix) const {
-   uint32_t want = dev->tuning.paths_per_wave ? dev->tuning.paths_per_wave : kDefaultPaths;
+uint32_t RenderEngine::capacity_for(uint64_t total_paths, uint32_t max_npix, bool regen) const {
+   uint32_t want = dev->tuning.paths_per_wave ? dev->tuning.paths_per_wave : regen ? kDefaultRegenPaths : kDefaultPaths;
    want = (uint32_t)std::min<uint64_t>(want, std::max<uint64_t>(total_paths, max_npix));
-   return std::max(std::max(want, max_npix), cap);
+   // regeneration keeps using what is allocated; wavefront queues are sized per call
+   return std::max(std::max(want, max_npix), regen ? cap : 0u);
 }
 
 int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vector<const ptx_scene *> &scenes,
@@ -254,8 +280,9 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
       max_npix = std::max(max_npix, j.npix);
       total_paths += (uint64_t)j.npix * j.desc.spp;
    }
-   const uint32_t paths = capacity_for(total_paths, max_npix);
-   const auto rounds = schedule_rounds(jobs, paths);
+   const bool regen = regen_round(1, (uint32_t)scenes.size());
+   const uint32_t paths = capacity_for(total_paths, max_npix, regen);
+   const auto rounds = schedule_rounds(jobs, paths, regen ? paths_max_chunks() : 0xFFFFFFFFu);
    size_t nchunks = 0;
    for (const auto &r : rounds) nchunks += r.size();
    int rc = ensure(paths, max_stack, nj, (uint32_t)scenes.size(), (uint32_t)nchunks);
@@ -286,7 +313,11 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
    for (const auto &r : rounds) {
       uint32_t depth = 0;
       for (const Chunk &c : r) depth = std::max(depth, jobs[c.job].desc.max_depth);
-      round(d_chunks + off, r, depth, (uint32_t)scenes.size());
+      if ((rc = round(d_chunks + off, r, depth, (uint32_t)scenes.size()))) {
+         (void)hipStreamSynchronize(st);
+         cleanup();
+         return rc;
+      }
       off += r.size();
    }
    for (uint32_t j = 0; j < nj; j++) {
@@ -499,6 +530,26 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
          }
       best.paths_per_wave = best_paths;
       best.render_mode = best_mode;
+      // 3. regeneration: lanes a wave gathers before it shades
+      if (best_mode == PTX_RENDER_REGEN) {
+         dev->tuning.paths_per_wave = best_paths;
+         dev->tuning.render_mode = best_mode;
+         uint32_t best_shade = 0;
+         for (uint32_t shade : {1u, 8u, 16u, 32u, 48u}) {
+            dev->tuning.shade_lanes = shade;
+            double t_min = 1e30;
+            for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
+               const double t0 = now_s();
+               rc = engine_of(dev)->run_batch(rj, {scene}, films);
+               t_min = std::min(t_min, now_s() - t0);
+            }
+            if (rc == PTX_OK && t_min < best_s) {
+               best_s = t_min;
+               best_shade = shade;
+            }
+         }
+         best.shade_lanes = best_shade;
+      }
    }
    (void)hipEventDestroy(e0);
    (void)hipEventDestroy(e1);

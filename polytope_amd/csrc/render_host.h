@@ -18,16 +18,21 @@ JobDev job_dev(const ptx_job_desc &j, uint32_t scene_index);
 // Device bytes a job holds while it runs (film + primary-hit records).
 size_t job_device_bytes(const ptx_job_desc &j);
 
-std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jobs, uint32_t cap);
+// Rounds of at most max_chunks chunks (one per job) sharing cap path slots.
+std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jobs, uint32_t cap,
+                                                uint32_t max_chunks);
 
 struct RenderEngine {
-   static constexpr uint32_t kDefaultPaths = 1u << 22;
+   static constexpr uint32_t kDefaultPaths = 1u << 22;        // wavefront rounds (172 B per slot)
+   static constexpr uint32_t kDefaultRegenPaths = 1u << 28;   // regeneration rounds (12 B per slot)
+   static constexpr uint32_t kDefaultShadeLanes = 16;   // lanes a wave gathers before it shades
    // layout of `words`: 8 traversal work counters, the two queue counts, the bounce index
    static constexpr uint32_t kWork = 0, kCountA = 8, kCountB = 9, kBounce = 10;
 
    ptx_device *dev = nullptr;
    hipStream_t stream = nullptr;
-   uint32_t cap = 0;                        // path slots
+   uint32_t cap = 0;                        // path slots (res)
+   uint32_t qcap = 0;                       // path slots of the wavefront queues (0 = none)
    float *qa = nullptr, *qb = nullptr;      // queues, 16 words per entry
    float *hits = nullptr;                   // 8 words per path slot
    float *res = nullptr;                    // 3 floats per path slot
@@ -43,20 +48,29 @@ struct RenderEngine {
    uint32_t chunks_cap = 0;
    uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0, paths_grid = 0;
    hipGraphExec_t g_ab = nullptr, g_ba = nullptr;
+   bool graphs_stale = true;
 
    ~RenderEngine();
    void release();
-   static size_t bytes_for(uint32_t paths);
+   // Device bytes of `paths` slots: results, plus queues and hit records when
+   // the wavefront form may run.
+   static size_t bytes_for(uint32_t paths, bool queues);
+   // Whether a round of nchunks chunks over nscenes scenes runs as one
+   // path-regeneration launch under the device's tuning.
+   bool regen_round(uint32_t nchunks, uint32_t nscenes) const;
    int ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uint32_t nscenes, uint32_t nchunks);
+   // Wavefront queues and hit records for at least `need` slots, and the
+   // per-bounce graphs.
+   int ensure_queues(uint32_t need);
    // Allocates the job's film and primary records, zeroes the film, traces the
    // primary hits.  J.film / J.primary are set.
    int start_job(JobDev &J, const DevScene &S, uint32_t npix);
    // One round over the chunks in d_round (a device copy of r); the job and
    // scene tables must hold the round's jobs, the scene table nscenes entries.
-   void round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes);
+   int round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes);
    int finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host);
    void free_job(JobDev &J);
-   uint32_t capacity_for(uint64_t total_paths, uint32_t max_npix) const;
+   uint32_t capacity_for(uint64_t total_paths, uint32_t max_npix, bool regen) const;
    int run_batch(const std::vector<RenderJob> &jobs, const std::vector<const ptx_scene *> &scenes,
                  const std::vector<float *> &films_host);
    int run_plain(const ptx_scene *scene, const ptx_job_desc &desc, float *film_host);

@@ -20,6 +20,7 @@
 #include <thread>
 #include <vector>
 
+#include "launch.h"
 #include "render_host.h"
 
 namespace ptx {
@@ -27,6 +28,7 @@ namespace {
 
 constexpr uint32_t kMaxServerJobs = 256;
 constexpr uint32_t kMaxServerScenes = 64;
+constexpr uint32_t kDefaultServerRegenPaths = 1u << 24;   // path slots per round (12 B each)
 
 struct JobState {
    uint64_t id = 0;
@@ -64,6 +66,7 @@ struct ptx_server {
    std::vector<std::shared_ptr<ptx::JobState>> running;
    size_t pool_bytes = 0, used_bytes = 0;
    uint32_t paths = 0;
+   size_t rotation = 0;         // first running job of the next round when more than fit
    std::vector<bool> slot_used;
    std::map<const ptx_scene *, std::pair<uint32_t, uint32_t>> scene_slots;   // scene -> (slot, refs)
    double t0 = 0;
@@ -167,7 +170,15 @@ void ptx_server::loop() {
          for (auto &j : running)
             if (!j->cancel) in_round.push_back(j);
       }
+      // at most paths_max_chunks() jobs per round, taken in rotation when more run
+      const size_t max_round = paths_max_chunks();
+      if (in_round.size() > max_round) {
+         std::rotate(in_round.begin(), in_round.begin() + (rotation % in_round.size()), in_round.end());
+         in_round.resize(max_round);
+         rotation += max_round;
+      }
       if (!in_round.empty()) {
+         int round_rc = PTX_OK;
          chunks.clear();
          uint64_t used = 0;
          uint32_t depth = 0;
@@ -185,7 +196,7 @@ void ptx_server::loop() {
          if (!chunks.empty()) {
             (void)hipMemcpyAsync(eng.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice,
                                  stream);
-            eng.round(eng.d_chunks, chunks, depth, kMaxServerScenes);
+            round_rc = eng.round(eng.d_chunks, chunks, depth, kMaxServerScenes);
          }
          const hipError_t e = hipStreamSynchronize(stream);
          std::lock_guard<std::mutex> lock(mu);
@@ -196,8 +207,8 @@ void ptx_server::loop() {
                   j->samples_done += (uint64_t)c.k * c.npix;
                }
          }
-         if (e != hipSuccess)
-            for (auto &j : in_round) fail(j, PTX_ERR_HIP);
+         if (e != hipSuccess || round_rc != PTX_OK)
+            for (auto &j : in_round) fail(j, round_rc != PTX_OK ? round_rc : PTX_ERR_HIP);
       }
       // retire finished and cancelled jobs
       std::vector<std::shared_ptr<JobState>> done;
@@ -242,16 +253,21 @@ int ptx_server_create(ptx_device *dev, const ptx_server_config *cfg, ptx_server 
    PTX_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
    s->eng.dev = dev;
    s->eng.stream = s->stream;
+   // rounds hold at most paths_max_chunks() jobs, so the default form is always
+   // path regeneration and only an explicit wavefront tuning needs queues
+   const bool queues = dev->tuning.render_mode == PTX_RENDER_WAVEFRONT;
    s->paths = cfg->paths_per_wave ? cfg->paths_per_wave
                                   : (dev->tuning.paths_per_wave ? dev->tuning.paths_per_wave
-                                                                : RenderEngine::kDefaultPaths);
+                                     : queues ? RenderEngine::kDefaultPaths : kDefaultServerRegenPaths);
    // the path pool is sized for scenes up to 64 levels deep; deeper scenes grow the spill
    int rc = s->eng.ensure(s->paths, 64, kMaxServerJobs, kMaxServerScenes, 4 * kMaxServerJobs);
+   if (rc == PTX_OK && queues) rc = s->eng.ensure_queues(s->paths);
    if (rc) {
+      s->eng.release();
       (void)hipStreamDestroy(s->stream);
       return rc;
    }
-   s->pool_bytes = RenderEngine::bytes_for(s->paths) + s->eng.spill_entries * sizeof(uint4);
+   s->pool_bytes = RenderEngine::bytes_for(s->paths, queues) + s->eng.spill_entries * sizeof(uint4);
    if (cfg->hbm_budget_bytes <= s->pool_bytes) {
       s->eng.release();
       (void)hipStreamDestroy(s->stream);

@@ -462,8 +462,11 @@ def stress_workload(n_rays: int = 1 << 24, spheres: int = 32, n_lat: int = 256, 
     features (vertices and float32 edge midpoints; half on the spheres, from
     origins 2.5 radii around the sphere's centre, half on the grid, from origins
     at distance 4 in random directions); the rest are incoherent (origin uniform in
-    the scene box, direction uniform).  Returns (meshes, rays, n_feature, t_target)
-    with t_target[i] = |target - origin| for the feature rays."""
+    the scene box, direction uniform).  Feature rays are never grazing: they
+    arrive within 53 degrees of the radial direction (spheres) or of the grid's
+    normal (grid targets off its border), so each one enters the surface at its
+    target and a watertight intersector hits at t <= |target - origin| (up to
+    rounding).  Returns (meshes, rays, n_feature, t_target)."""
     rng = Rng(seed)
     sv, sf = latlong_sphere(n_lat, n_lon, 1.0)
     unit = sv.astype(np.float64)
@@ -494,15 +497,28 @@ def stress_workload(n_rays: int = 1 << 24, spheres: int = 32, n_lat: int = 256, 
         mid = ((a + b) * np.float32(0.5)).astype(np.float32)
         return np.where((np.arange(n) % 2 == 0)[:, None], a, mid)
 
+    def cone(axis, n):
+        """Unit vectors within asin(0.8) (53 degrees) of the unit rows of ``axis``."""
+        d = axis + 0.8 * unit_vectors(n, rng)
+        return d / np.linalg.norm(d, axis=1)[:, None]
+
     which = (rng.uniform(n_sph) * spheres).astype(np.int64) % spheres
     tgt_s = np.zeros((n_sph, 3), np.float32)
     org_s = np.zeros((n_sph, 3), np.float64)
     for i in range(spheres):
         sel = np.nonzero(which == i)[0]
         tgt_s[sel] = targets(meshes[i].verts, sf, len(sel))
-        org_s[sel] = centers[i] + unit_vectors(len(sel), rng) * 2.5
-    tgt_g = targets(gv, gf, n_grid)
-    org_g = tgt_g.astype(np.float64) + unit_vectors(n_grid, rng) * 4.0
+        radial = tgt_s[sel].astype(np.float64) - centers[i]
+        radial /= np.linalg.norm(radial, axis=1)[:, None]
+        org_s[sel] = tgt_s[sel] + cone(radial, len(sel)) * 2.5
+    # grid targets: features of cells off the border (a border edge has one face)
+    cell = np.arange(gf.shape[0]) % (grid_n * grid_n)
+    ci_, cj_ = cell // grid_n, cell % grid_n
+    inner = np.nonzero((ci_ >= 1) & (ci_ <= grid_n - 2) & (cj_ >= 1) & (cj_ <= grid_n - 2))[0]
+    tgt_g = targets(gv, gf[inner], n_grid)
+    up = np.zeros((n_grid, 3))
+    up[:, 1] = np.where(rng.uniform(n_grid) < 0.5, -1.0, 1.0)     # from above or below the grid
+    org_g = tgt_g.astype(np.float64) + cone(up, n_grid) * 4.0
     tgt = np.concatenate([tgt_s, tgt_g], 0).astype(np.float64)
     org = np.concatenate([org_s, org_g], 0).astype(np.float32).astype(np.float64)
     d = tgt - org

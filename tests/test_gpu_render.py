@@ -125,14 +125,16 @@ def test_render_batch_alone_equals_mixed(dev, mode):
     assert np.array_equal(mixed[1].view(np.uint32), ref.view(np.uint32))
 
 
-def test_render_batch_more_chunks_than_lds_table(dev):
-    # 70 jobs in one round: more chunks than the regeneration kernel's LDS table,
-    # so the default mode runs that round in wavefront form; films are unchanged
+@pytest.mark.parametrize("mode", MODES)
+def test_render_batch_more_jobs_than_lds_table(dev, mode):
+    # 70 jobs: more than the regeneration kernel's 64-chunk LDS table, so its rounds
+    # take at most 64 jobs each (the wavefront form takes all 70 in one round);
+    # films are unchanged
     meshes, cam, sky = _lambert_scene(16, 12)
     sc = Scene(dev, meshes, sky)
     jobs = [(sc, cam, 1 + i % 3, 1 + i % 4, 100 + i) for i in range(70)]
-    print("case 70 jobs", flush=True)
-    mixed = render_batch(jobs)
-    for i in (0, 1, 33, 69):
+    print(f"case 70 jobs mode={mode}", flush=True)
+    mixed = batch(dev, jobs, mode)
+    for i in (0, 1, 33, 64, 69):
         (alone,) = render_batch([jobs[i]])
         assert np.array_equal(alone.view(np.uint32), mixed[i].view(np.uint32))

@@ -63,6 +63,23 @@ def test_server_results_equal_alone(dev, mode):
         dev.set_tuning()
 
 
+def test_server_more_jobs_than_a_round_holds(dev):
+    # 70 jobs in flight: rounds take 64 of them in rotation; every film equals the
+    # same job rendered alone
+    sa, cam_a, sb, _ = _scenes(dev)
+    small = S.Camera(cam_a.camera_to_world, 60.0, 16, 12)
+    jobs = [(sa, small, 2 + i % 3, 1 + i % 4, 300 + i) for i in range(70)]
+    srv = Server(dev, hbm_budget_bytes=8 << 30, max_jobs_in_flight=128, paths_per_wave=20000)
+    try:
+        ids = [srv.submit(*j) for j in jobs]
+        films = [srv.wait(i)[0] for i in ids]
+    finally:
+        srv.close()
+    for i in (0, 5, 63, 64, 69):
+        (alone,) = render_batch([jobs[i]])
+        assert np.array_equal(alone.view(np.uint32), films[i].view(np.uint32))
+
+
 def test_server_budget_admission_and_rejection(dev):
     sa, cam_a, sb, _ = _scenes(dev)
     # a budget smaller than the path pool itself

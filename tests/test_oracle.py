@@ -311,3 +311,18 @@ def test_closest_hit_against_brute_force():
         assert abs(got["t"][i] - tt[order[0]]) <= 1e-5 * max(1.0, tt[order[0]])
         checked += 1
     assert checked > 2500
+
+
+def test_stress_feature_rays_do_not_leak():
+    # the stress workload at reduced size (4 displaced spheres of 4,096 faces and a
+    # 64x64 sliver grid): every ray aimed exactly at a shared vertex or edge
+    # midpoint, arriving non-grazing, hits at or before its target
+    meshes, rays, n_feat, t_target = S.stress_workload(1 << 18, spheres=4, n_lat=32, n_lon=64, grid_n=64)
+    ref = O.OracleScene(meshes).intersect(rays[:n_feat])
+    hit = ref["hit"] == 1
+    assert hit.all()
+    assert (ref["t"] <= t_target * (1 + 1e-5) + 1e-6).all()
+    # a BVH handed in from outside gives the same hits as the restated build
+    sc = O.OracleScene(meshes)
+    again = O.OracleScene(meshes, bvh=sc.bvh()).intersect(rays[:4096])
+    assert np.array_equal(again.view(np.uint8), ref[:4096].view(np.uint8))
