@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--lds", type=int, default=0, help="LDS stack entries per lane (0 = library default)")
     ap.add_argument("--refill", type=int, default=0, help="idle lanes that trigger a refill (0 = default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--leaf-lanes", type=int, default=0,
+                    help="lanes a traversal wave gathers at leaves before a leaf step (0 = default)")
     ap.add_argument("--no-coherent", action="store_true", help="skip the coherent-rate measurement")
     ap.add_argument("--skip-render", action="store_true", help="skip the plain vs batched render comparison")
     ap.add_argument("--skip-serving", action="store_true", help="skip the Poisson serving run")
@@ -111,13 +113,13 @@ def render_section(dev, args):
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     samples = sum(c.width * c.height * spp for c, spp, _, _ in jobs)
     batch = [(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
-    dev.set_tuning(shade_lanes=args.shade_lanes, paths_per_wave=args.render_paths)
+    dev.set_tuning(shade_lanes=args.shade_lanes, paths_per_wave=args.render_paths, leaf_lanes=args.leaf_lanes)
     render_batch([(scene, jobs[0][0], 1, 8, 1)])                  # warm the engine and graphs
     tb = time.perf_counter()
     films_b = render_batch(batch)
     t_batched = time.perf_counter() - tb
     print(f"[bench] batched: {samples / t_batched / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
-    dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT, paths_per_wave=args.render_paths)
+    dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT, paths_per_wave=args.render_paths, leaf_lanes=args.leaf_lanes)
     render_batch([(scene, jobs[0][0], 1, 8, 1)])
     tw = time.perf_counter()
     films_w = render_batch(batch)
@@ -190,8 +192,9 @@ def main():
     print(f"[bench] workload built: {meshes[0].num_faces} triangles, {n_rays} rays "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     dev = Device(0)
-    if args.lds or args.refill or args.blocks_per_cu:
-        dev.set_tuning(lds_stack=args.lds, refill_lanes=args.refill, blocks_per_cu=args.blocks_per_cu)
+    if args.lds or args.refill or args.blocks_per_cu or args.leaf_lanes:
+        dev.set_tuning(lds_stack=args.lds, refill_lanes=args.refill, blocks_per_cu=args.blocks_per_cu,
+                       leaf_lanes=args.leaf_lanes)
     sc = Scene(dev, meshes)
     info = sc.info()
     print(f"[bench] scene: {info['num_nodes']} nodes, build {info['build_seconds']:.2f} s, "

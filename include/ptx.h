@@ -188,6 +188,9 @@ typedef struct ptx_tuning {
                              /* its LDS tables (64 chunks, 64 scenes), else wavefront */
    uint32_t shade_lanes;     /* regeneration: lanes of a wave that must wait for a    */
                              /* shading step before the wave shades (1..64; 0 = 16)  */
+   uint32_t leaf_lanes;      /* traversal: lanes of a wave that must reach a leaf     */
+                             /* before the wave runs a leaf step (1..64; 0 = 1 for   */
+                             /* ray queries, 16 for rendering)                       */
 } ptx_tuning;
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);

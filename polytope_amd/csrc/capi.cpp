@@ -46,6 +46,11 @@ int upload(ptx_scene *s, const std::vector<T> &v, const T **dst) {
 
 uint32_t lds_depth_of(const ptx_device *dev) { return dev->tuning.lds_stack ? dev->tuning.lds_stack : 12; }
 
+uint32_t leaf_lanes_of(const ptx_device *dev, bool render) {
+   if (dev->tuning.leaf_lanes) return std::min(64u, dev->tuning.leaf_lanes);
+   return render ? kDefaultRenderLeafLanes : kDefaultLeafLanes;
+}
+
 uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds) {
    int per_cu = closest_hit_blocks_per_cu(lds);
    if (dev->tuning.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)dev->tuning.blocks_per_cu);
@@ -70,7 +75,8 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
    uint32_t *work = static_cast<uint32_t *>(scratch);
    uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(scratch) + 256);
    const uint32_t refill = dev->tuning.refill_lanes ? std::min(64u, dev->tuning.refill_lanes) : 1u;
-   const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill,
+   const uint32_t leaf = leaf_lanes_of(dev);
+   const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill, leaf,
                                      stream);
    const hipError_t fe = hipFreeAsync(scratch, stream);
    if (rc != 0) return set_error(PTX_ERR_HIP, "closest-hit launch failed");
@@ -168,7 +174,7 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    std::copy(img.root_box, img.root_box + 6, s->d.root_box);
    s->d.root_ref = img.root_ref;
    s->d.root_count = img.root_count;
-   s->d.num_tris = (uint32_t)(img.tris.size() / 9);
+   s->d.num_tris = img.num_tris;
    s->d.max_stack = img.max_stack;
    s->d.has_sky = desc->has_skybox;
    std::copy(desc->skybox, desc->skybox + 3, s->d.sky);
@@ -266,6 +272,7 @@ int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
       return set_error(PTX_ERR_INVALID, "lds_stack must be 0, 8, 12, 16 or 24");
    if (t->render_mode > PTX_RENDER_WAVEFRONT) return set_error(PTX_ERR_INVALID, "render_mode must be 0, 1 or 2");
    if (t->shade_lanes > 64) return set_error(PTX_ERR_INVALID, "shade_lanes must be 0..64");
+   if (t->leaf_lanes > 64) return set_error(PTX_ERR_INVALID, "leaf_lanes must be 0..64");
    dev->tuning = *t;
    return PTX_OK;
 }
@@ -276,6 +283,7 @@ int ptx_get_tuning(ptx_device *dev, ptx_tuning *t) {
    if (!t->lds_stack) t->lds_stack = lds_depth_of(dev);
    if (!t->block_size) t->block_size = 256;
    if (!t->refill_lanes) t->refill_lanes = 1;
+   // leaf_lanes stays 0 when unset: each kernel family then uses its own default
    if (!t->shade_lanes) t->shade_lanes = RenderEngine::kDefaultShadeLanes;
    return PTX_OK;
 }

@@ -10,7 +10,8 @@ namespace ptx {
 
 int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits,
                        ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
-                       uint32_t grid, uint32_t lds_depth, uint32_t refill_min, hipStream_t stream);
+                       uint32_t grid, uint32_t lds_depth, uint32_t refill_min, uint32_t leaf_min,
+                       hipStream_t stream);
 int closest_hit_blocks_per_cu(uint32_t lds_depth);
 
 }  // namespace ptx
@@ -19,13 +20,15 @@ namespace ptx {
 
 uint32_t render_lds_depth();
 int render_blocks_per_cu(int which);   // 0 extend, 1 primary, 2 plain megakernel
-void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t grid, hipStream_t st);
+void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t leaf_min, uint32_t grid,
+                    hipStream_t st);
 void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs,
                         const DevScene *scenes, float *res, float *q_out, uint32_t *count_out, uint32_t *bounce,
                         uint32_t grid, hipStream_t st);
 void launch_bounce(const float *q_in, const uint32_t *count_in, float *q_out, uint32_t *count_out, float *hits,
                    const JobDev *jobs, const DevScene *scenes, float *res, uint32_t *work, uint4 *spill,
-                   uint32_t max_stack, uint32_t *bounce, uint32_t ext_grid, uint32_t shade_grid, hipStream_t st);
+                   uint32_t max_stack, uint32_t leaf_min, uint32_t *bounce, uint32_t ext_grid, uint32_t shade_grid,
+                   hipStream_t st);
 void launch_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs, const float *res, uint32_t grid,
                        hipStream_t st);
 void launch_finalize(float *film, uint32_t npix, uint32_t spp, hipStream_t st);
@@ -37,7 +40,7 @@ uint32_t paths_max_chunks();
 uint32_t paths_max_scenes();
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-                  uint32_t grid, hipStream_t st);
+                  uint32_t leaf_min, uint32_t grid, hipStream_t st);
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st);
 
 }  // namespace ptx

@@ -250,9 +250,9 @@ struct PrimaryPolicy {
 };
 
 template <int kLds>
-__global__ void __launch_bounds__(256) k_primary(DevScene S, JobDev J, uint32_t *work, uint4 *spill) {
+__global__ void __launch_bounds__(256) k_primary(DevScene S, JobDev J, uint32_t *work, uint4 *spill, uint32_t leaf_min) {
    const PrimaryPolicy P{nullptr, S, J};
-   persistent_trace<kLds>(P, J.width * J.height, work, spill, 1u, S.max_stack);
+   persistent_trace<kLds>(P, J.width * J.height, work, spill, 1u, leaf_min, S.max_stack);
 }
 
 // ---------------------------------------------------------------- path queues
@@ -405,10 +405,10 @@ struct ExtendPolicy {
 template <int kLds>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_extend(const float *queue, const uint32_t *count, const DevScene *scenes, float *hits, uint32_t *work, uint4 *spill,
-         uint32_t max_stack) {
+         uint32_t max_stack, uint32_t leaf_min) {
    const ExtendPolicy P{queue, scenes, hits};
    const uint32_t n = __builtin_amdgcn_readfirstlane(*count);
-   persistent_trace<kLds>(P, n, work, spill, 1u, max_stack);
+   persistent_trace<kLds>(P, n, work, spill, 1u, leaf_min, max_stack);
 }
 
 // Shade stage at bounce b >= 1 (PathTraceIntegrator.cpp:23-95).
@@ -630,7 +630,8 @@ __device__ __forceinline__ uint32_t p_root(const DevScene &S, Trav &s, float *re
 template <int kLds>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
-        uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min) {
+        uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
+        uint32_t leaf_min) {
    __shared__ uint32_t lds_ref[kLds * 256];
    __shared__ float lds_near[kLds * 256];
    __shared__ uint16_t lds_cnt[kLds * 256];
@@ -688,7 +689,12 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
       const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3;
       const bool run_heavy =
             (uint32_t)__popcll(__ballot(heavy)) >= shade_min || !__any(m0 == kPTrace || m0 == kPShade1);
-      const uint32_t m = heavy && !run_heavy ? kPHold : m0;
+      // leaf steps gather the same way; lanes at interior nodes or in Shade1
+      // always step, so some lane progresses whenever any lane holds
+      const bool at_leaf = m0 == kPTrace && (s.cur & kLeafBit) != 0;
+      const bool run_leaf = (uint32_t)__popcll(__ballot(at_leaf)) >= leaf_min ||
+                            !__any((m0 == kPTrace && !at_leaf) || m0 == kPShade1);
+      const uint32_t m = (heavy && !run_heavy) || (at_leaf && !run_leaf) ? kPHold : m0;
       // ------------------------------------------------------------ fetch
       Fetched f;
       if (m == kPTrace) {
@@ -698,24 +704,25 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          mode = kPPrim | (cf << kPChunkShift);
          const LdsChunk &c = sch[cf];
          const uint32_t pixel = (slot - c.base) % c.npix;
-         const float4 *pr = reinterpret_cast<const float4 *>(c.primary + 16 * (size_t)pixel);
-         const float4 q0 = pr[0], q1 = pr[1], q2 = pr[2], q3 = pr[3];
+         const PTX_GLOBAL f32x4u *pr = as_global(reinterpret_cast<const f32x4u *>(c.primary + 16 * (size_t)pixel));
+         const f32x4u q0 = pr[0], q1 = pr[1], q2 = pr[2], q3 = pr[3];
          f.q[0] = q0.x; f.q[1] = q0.y; f.q[2] = q0.z; f.q[3] = q0.w;
          f.q[4] = q1.x; f.q[5] = q1.y; f.q[6] = q1.z; f.q[7] = q1.w;
          f.q[8] = q2.x; f.q[9] = q2.y; f.q[10] = q2.z; f.q[11] = q2.w;
          f.q[12] = q3.x; f.q[13] = q3.y; f.q[14] = q3.z; f.q[15] = q3.w;
       } else if (m == kPShade1) {
-         f.q[0] = __uint_as_float(ssc[sch[lw_chunk(mode)].scene].pairs[2 * (size_t)s.hit]);
+         f.q[0] = __uint_as_float(as_global(ssc[sch[lw_chunk(mode)].scene].pairs)[2 * (size_t)s.hit]);
       } else if (m == kPShade2) {
-         const uint32_t *mr = reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0);
+         const PTX_GLOBAL uint32_t *mr =
+               as_global(reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0));
 #pragma unroll
          for (int k = 0; k < 9; k++) f.q[k] = __uint_as_float(mr[k]);
-         const float *v = tris + 9 * (size_t)s.hit;
+         const PTX_GLOBAL float *v = as_global(tris + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) f.q[9 + k] = v[k];
       } else if (m == kPShade3) {
-         const float *v = tris + 9 * (size_t)s.hit;
-         const float *nrm = ssc[sch[lw_chunk(mode)].scene].normals + 9 * (size_t)s.hit;
+         const PTX_GLOBAL float *v = as_global(tris + 9 * (size_t)s.hit);
+         const PTX_GLOBAL float *nrm = as_global(ssc[sch[lw_chunk(mode)].scene].normals + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) { f.q[k] = v[k]; f.q[9 + k] = nrm[k]; }
       }
@@ -739,7 +746,12 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             mode = kPIdle;
          } else {
             const DevScene &S = ssc[c.scene];
-            const MeshRec &M = S.meshes[w0 >> 2];
+            const PTX_GLOBAL uint32_t *mw = as_global(reinterpret_cast<const uint32_t *>(S.meshes + (w0 >> 2)));
+            MeshRec M;
+            M.brdf = (int32_t)mw[0];
+            M.refl[0] = __uint_as_float(mw[1]);
+            M.refl[1] = __uint_as_float(mw[2]);
+            M.refl[2] = __uint_as_float(mw[3]);
             Frame fr;
             fr.lx = f.q[1]; fr.ly = f.q[2]; fr.lz = f.q[3];
             fr.t1x = f.q[4]; fr.t1y = f.q[5]; fr.t1z = f.q[6];
@@ -837,10 +849,10 @@ int render_blocks_per_cu(int which) {
    return e == hipSuccess && b > 0 ? b : 1;
 }
 
-void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t grid,
+void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t leaf_min, uint32_t grid,
                     hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
-   hipLaunchKernelGGL(dev::k_primary<kRenderLds>, dim3(grid), dim3(256), 0, st, S, J, work, spill);
+   hipLaunchKernelGGL(dev::k_primary<kRenderLds>, dim3(grid), dim3(256), 0, st, S, J, work, spill, leaf_min);
 }
 
 void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs,
@@ -855,10 +867,11 @@ void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, 
 // replayed for every bounce.
 void launch_bounce(const float *q_in, const uint32_t *count_in, float *q_out, uint32_t *count_out, float *hits,
                    const JobDev *jobs, const DevScene *scenes, float *res, uint32_t *work, uint4 *spill,
-                   uint32_t max_stack, uint32_t *bounce, uint32_t ext_grid, uint32_t shade_grid, hipStream_t st) {
+                   uint32_t max_stack, uint32_t leaf_min, uint32_t *bounce, uint32_t ext_grid, uint32_t shade_grid,
+                   hipStream_t st) {
    hipLaunchKernelGGL(dev::k_bounce_prologue, dim3(1), dim3(64), 0, st, work, count_out, bounce);
    hipLaunchKernelGGL(dev::k_extend<kRenderLds>, dim3(ext_grid), dim3(256), 0, st, q_in, count_in, scenes, hits, work,
-                      spill, max_stack);
+                      spill, max_stack, leaf_min);
    hipLaunchKernelGGL(dev::k_shade, dim3(shade_grid), dim3(256), 0, st, q_in, count_in, hits, jobs, scenes, res,
                       dev::QueueRefs{q_out, count_out}, bounce);
 }
@@ -889,10 +902,10 @@ uint32_t paths_max_scenes() { return dev::kMaxRoundScenes; }
 
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-                  uint32_t grid, hipStream_t st) {
+                  uint32_t leaf_min, uint32_t grid, hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
    hipLaunchKernelGGL(dev::k_paths<kRenderLds>, dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, scenes,
-                      nscenes, res, work, spill, max_stack, shade_min);
+                      nscenes, res, work, spill, max_stack, shade_min, leaf_min);
 }
 
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {

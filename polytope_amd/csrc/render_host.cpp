@@ -163,6 +163,10 @@ int RenderEngine::ensure_queues(uint32_t need) {
       qcap = need;
       graphs_stale = true;
    }
+   if (graph_leaf != leaf_lanes_of(dev, true)) {
+      graph_leaf = leaf_lanes_of(dev, true);
+      graphs_stale = true;
+   }
    if (graphs_stale || !g_ab) {
       if (g_ab) (void)hipGraphExecDestroy(g_ab);
       if (g_ba) (void)hipGraphExecDestroy(g_ba);
@@ -173,7 +177,7 @@ int RenderEngine::ensure_queues(uint32_t need) {
          float *qin = dir == 0 ? qa : qb, *qout = dir == 0 ? qb : qa;
          uint32_t *cin = words + (dir == 0 ? kCountA : kCountB), *cout = words + (dir == 0 ? kCountB : kCountA);
          launch_bounce(qin, cin, qout, cout, hits, d_jobs, d_scenes, res, words + kWork, spill, stack_cap,
-                       words + kBounce, ext_grid, shade_grid, stream);
+                       leaf_lanes_of(dev, true), words + kBounce, ext_grid, shade_grid, stream);
          PTX_HIP_TRY(hipStreamEndCapture(stream, &g));
          hipGraphExec_t ex = nullptr;
          const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -228,7 +232,8 @@ int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
       return rc;
    }
    launch_zero_floats(J.film, (size_t)npix * 3, stream);
-   launch_primary(S, J, words + kWork, spill, std::min<uint32_t>(ext_grid, (npix + 255) / 256), stream);
+   launch_primary(S, J, words + kWork, spill, leaf_lanes_of(dev, true), std::min<uint32_t>(ext_grid, (npix + 255) / 256),
+                  stream);
    return PTX_OK;
 }
 
@@ -238,7 +243,7 @@ int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint3
    if (regen_round((uint32_t)r.size(), nscenes)) {
       const uint32_t shade_min = dev->tuning.shade_lanes ? dev->tuning.shade_lanes : kDefaultShadeLanes;
       launch_paths(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, nscenes, res, words + kWork, spill, stack_cap,
-                   shade_min, std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
+                   shade_min, leaf_lanes_of(dev, true), std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
       launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
       return PTX_OK;
    }

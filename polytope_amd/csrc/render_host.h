@@ -49,6 +49,7 @@ struct RenderEngine {
    uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0, paths_grid = 0;
    hipGraphExec_t g_ab = nullptr, g_ba = nullptr;
    bool graphs_stale = true;
+   uint32_t graph_leaf = 0;                 // leaf_min the graphs were captured with
 
    ~RenderEngine();
    void release();

@@ -134,7 +134,9 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
       img.root_box[k] = bvh.nodes[0].p0[k];
       img.root_box[3 + k] = bvh.nodes[0].p1[k];
    }
-   img.tris.resize(9 * np);
+   // 16 floats of padding: a traversal step loads 64 bytes from a leaf's first face
+   img.tris.assign(9 * np + 16, 0.f);
+   img.num_tris = (uint32_t)np;
    bool any_normals = false;
    for (const auto &m : meshes) any_normals |= m.has_normals;
    if (any_normals) img.normals.assign(9 * np, 0.f);

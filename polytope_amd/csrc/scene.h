@@ -33,6 +33,7 @@ struct HostImage {
    uint32_t root_ref = 0, root_count = 0;
    uint32_t max_stack = 1;
    uint32_t num_interior = 0, num_leaves = 0;
+   uint32_t num_tris = 0;
 };
 
 int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostImage &img);
@@ -40,6 +41,12 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
 double now_s();
 const char *last_error_cstr();
 uint32_t lds_depth_of(const ptx_device *dev);
+// Lanes of a wave that gather at leaves before the wave runs a leaf step:
+// measured best 1 for ray queries (leaf and interior steps are balanced) and
+// 16 for the render kernels (whose leaf and shading steps share the wave).
+constexpr uint32_t kDefaultLeafLanes = 1;
+constexpr uint32_t kDefaultRenderLeafLanes = 16;
+uint32_t leaf_lanes_of(const ptx_device *dev, bool render = false);
 uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds);
 size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds);
 int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,

@@ -70,9 +70,10 @@ struct RayQuery {
 template <int kLds>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n, ptx_hit *__restrict__ hits,
-              ptx_ray_counters *counters, ptx_hit_frame *frames, uint32_t *work, uint4 *spill, uint32_t refill_min) {
+              ptx_ray_counters *counters, ptx_hit_frame *frames, uint32_t *work, uint4 *spill, uint32_t refill_min,
+              uint32_t leaf_min) {
    const RayQuery q{sc, rays, hits, counters, frames};
-   persistent_trace<kLds>(q, n, work, spill, refill_min, sc.max_stack);
+   persistent_trace<kLds>(q, n, work, spill, refill_min, leaf_min, sc.max_stack);
 }
 
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
@@ -86,25 +87,26 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 // for (max_stack - lds) * grid * 256 entries.
 int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits,
                        ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
-                       uint32_t grid, uint32_t lds_depth, uint32_t refill_min, hipStream_t stream) {
+                       uint32_t grid, uint32_t lds_depth, uint32_t refill_min, uint32_t leaf_min,
+                       hipStream_t stream) {
    hipLaunchKernelGGL(dev::k_zero_u32, dim3(1), dim3(64), 0, stream, d_work, (uint32_t)dev::kSegments);
    if (n == 0) return 0;
    switch (lds_depth) {
       case 8:
          hipLaunchKernelGGL(dev::k_closest_hit<8>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min);
+                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
          break;
       case 16:
          hipLaunchKernelGGL(dev::k_closest_hit<16>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min);
+                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
          break;
       case 24:
          hipLaunchKernelGGL(dev::k_closest_hit<24>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min);
+                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
          break;
       default:
          hipLaunchKernelGGL(dev::k_closest_hit<12>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min);
+                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
          break;
    }
    return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -98,22 +98,41 @@ struct Fetched {
    float q[18];
 };
 
+// 16- and 8-byte loads at 4-byte alignment (faces are 36 bytes apart).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f32x2u __attribute__((ext_vector_type(2), aligned(4)));
+
+// Pointers read from memory (scene tables, LDS) are generic; loads through
+// them would be flat loads, which also count against the LDS counter and so
+// make every later LDS access wait for them.  Scene arrays live in device
+// memory, so they are read through global-address-space pointers.
+#define PTX_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const PTX_GLOBAL T *as_global(const T *p) {
+   return (const PTX_GLOBAL T *)p;
+}
+
+// One load shape for both kinds of node: 64 bytes from a per-lane address (the
+// interior record, or the leaf's first face and the start of its second), plus
+// 8 bytes for a leaf's second face.  The operands land in the same registers
+// whatever the kind, so no lane waits on its loads before the compute phase.
+// A one-face leaf reads 28 bytes past its face: the face array carries 64
+// bytes of padding.
 __device__ __forceinline__ void fetch(const Trav &s, const Node64 *nodes, const float *tris, Fetched &f) {
-   if (s.cur & kLeafBit) {
-      const float *p = tris + 9 * (size_t)(s.cur & ~kLeafBit);
-#pragma unroll
-      for (int k = 0; k < 9; k++) f.q[k] = p[k];
-      if (s.cnt > 1) {
-#pragma unroll
-         for (int k = 0; k < 9; k++) f.q[9 + k] = p[9 + k];
-      }
-   } else {
-      const float4 *rec = reinterpret_cast<const float4 *>(nodes + s.cur);
-      const float4 a = rec[0], b = rec[1], c = rec[2], m = rec[3];
-      f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
-      f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
-      f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
-      f.q[12] = m.x; f.q[13] = m.y; f.q[14] = m.z; f.q[15] = m.w;
+   const bool leaf = (s.cur & kLeafBit) != 0;
+   const PTX_GLOBAL float *p = as_global(leaf ? tris + 9 * (size_t)(s.cur & ~kLeafBit)
+                                              : reinterpret_cast<const float *>(nodes + s.cur));
+   const f32x4u a = *reinterpret_cast<const PTX_GLOBAL f32x4u *>(p);
+   const f32x4u b = *reinterpret_cast<const PTX_GLOBAL f32x4u *>(p + 4);
+   const f32x4u c = *reinterpret_cast<const PTX_GLOBAL f32x4u *>(p + 8);
+   const f32x4u d = *reinterpret_cast<const PTX_GLOBAL f32x4u *>(p + 12);
+   f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
+   f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
+   f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
+   f.q[12] = d.x; f.q[13] = d.y; f.q[14] = d.z; f.q[15] = d.w;
+   if (leaf && s.cnt > 1) {
+      const f32x2u e = *reinterpret_cast<const PTX_GLOBAL f32x2u *>(p + 16);
+      f.q[16] = e.x; f.q[17] = e.y;
    }
 }
 
@@ -218,7 +237,7 @@ struct RaySource {
    }
 };
 
-enum : uint32_t { kIdle = 0, kPending = 1, kStarting = 2, kActive = 3 };
+enum : uint32_t { kIdle = 0, kPending = 1, kStarting = 2, kActive = 3, kHeld = 4 };
 
 // Runs the persistent loop.  Policy supplies:
 //   Ctx                                   per-lane work context
@@ -231,9 +250,13 @@ enum : uint32_t { kIdle = 0, kPending = 1, kStarting = 2, kActive = 3 };
 //   nodes(ctx), tris(ctx)                 the lane's scene arrays
 //   debug_pixel(ctx)                      reference debug-pixel quirk flag
 //   finish(id, ctx, s)                    write the result
+// A leaf step (up to two Wald tests) costs several interior steps and runs for
+// the whole wave when any lane takes it, so lanes whose next node is a leaf
+// hold until at least leaf_min of them wait or no other lane can step; every
+// lane still performs its own steps in its own order.
 template <int kLds, class Policy>
 __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, uint32_t *work, uint4 *spill,
-                                                 uint32_t refill_min, uint32_t max_stack) {
+                                                 uint32_t refill_min, uint32_t leaf_min, uint32_t max_stack) {
    __shared__ uint32_t lds_ref[kLds * 256];
    __shared__ float lds_near[kLds * 256];
    __shared__ uint16_t lds_cnt[kLds * 256];
@@ -265,15 +288,19 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
          }
       }
       if (!__any(mode != kIdle)) break;
+      const bool at_leaf = mode == kActive && (s.cur & kLeafBit) != 0;
+      const bool hold = at_leaf && (uint32_t)__popcll(__ballot(at_leaf)) < leaf_min &&
+                        __any(mode != kIdle && !at_leaf);
+      const uint32_t m = hold ? kHeld : mode;
       Fetched f;
-      if (mode == kPending) {
+      if (m == kPending) {
          P.load(id, f);
-      } else if (mode == kStarting) {
+      } else if (m == kStarting) {
          P.load2(ctx, f);
-      } else if (mode == kActive) {
+      } else if (m == kActive) {
          fetch(s, P.nodes(ctx), P.tris(ctx), f);
       }
-      if (mode == kPending) {
+      if (m == kPending) {
          const int r = P.start(id, f, ctx, s);
          if (r == 2) {
             P.finish(id, ctx, s);
@@ -288,14 +315,14 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
                mode = kIdle;
             }
          }
-      } else if (mode == kStarting) {
+      } else if (m == kStarting) {
          P.start2(ctx, f, s);
          mode = kActive;
          if (s.cur == 0xFFFFFFFFu) {
             P.finish(id, ctx, s);
             mode = kIdle;
          }
-      } else if (mode == kActive) {
+      } else if (m == kActive) {
          if (!advance<kLds>(s, f, st, P.debug_pixel(ctx))) {
             P.finish(id, ctx, s);
             mode = kIdle;
