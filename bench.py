@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--subdiv", type=int, default=8)
     ap.add_argument("--rays-log2", type=int, default=22)
-    ap.add_argument("--cpu-sample-log2", type=int, default=17)
+    ap.add_argument("--cpu-sample-log2", type=int, default=21)
+    ap.add_argument("--cpu-repeat", type=int, default=36, help="passes over the CPU sample (about 10 s)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--lds", type=int, default=0, help="LDS stack entries per lane (0 = library default)")
@@ -72,29 +73,50 @@ def algorithmic_bytes(counters: np.ndarray) -> np.ndarray:
     return 24.0 + 20.0 + 32.0 * n_box + 36.0 * counters["tri_isects"].astype(np.float64)
 
 
-def cpu_baseline(meshes, rays: np.ndarray, threads: int):
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(meshes, rays: np.ndarray, threads: int, repeat: int):
     import oracle as O
     from polytope_amd import scenes as S
     if O.reference_available():
         with tempfile.TemporaryDirectory() as d:
             S.write_ptxm(f"{d}/s.bin", meshes)
             S.write_rays(f"{d}/r.bin", rays)
-            p = subprocess.run([str(O.REF_DRIVER), "--ptxm", f"{d}/s.bin", "--time-rays", f"{d}/r.bin",
-                                "--threads", str(threads)], capture_output=True, text=True, timeout=900, cwd=d)
-            line = [ln[ln.index("{\"rays\""):] for ln in p.stdout.splitlines() if "{\"rays\"" in ln]
-            if p.returncode == 0 and line:
-                r = json.loads(line[-1])
+            S.write_rays(f"{d}/r1.bin", rays[: 1 << 16])
+
+            def timed(path, nthreads, reps):
+                p = subprocess.run([str(O.REF_DRIVER), "--ptxm", f"{d}/s.bin", "--time-rays", path,
+                                    "--threads", str(nthreads), "--repeat", str(reps)],
+                                   capture_output=True, text=True, timeout=900, cwd=d)
+                line = [ln[ln.index("{\"rays\""):] for ln in p.stdout.splitlines() if "{\"rays\"" in ln]
+                return json.loads(line[-1]) if p.returncode == 0 and line else None
+
+            r = timed(f"{d}/r.bin", threads, repeat)
+            r1 = timed(f"{d}/r1.bin", 1, 1)
+            if r:
                 return {"value": round(r["mrays_per_s"], 4), "unit": "Mrays/s", "cores": threads,
                         "kind": "reference",
-                        "sample": f"first {len(rays)} incoherent rays of the workload, reference "
-                                  f"bvh::intersect_compact, {threads} threads over contiguous ray chunks"}
+                        "sample": f"the workload's {len(rays)} incoherent rays x {repeat} repeats "
+                                  f"({r['seconds']:.1f} s), reference bvh::intersect_compact built from "
+                                  f"its own sources, {threads} threads over contiguous ray chunks",
+                        "one_thread_mrays_per_s": round(r1["mrays_per_s"], 4) if r1 else None,
+                        "cpu_model": cpu_model()}
     sc = O.OracleScene(meshes)
     sc.intersect_ids(rays[: min(len(rays), 4096)], threads)
     t0 = time.perf_counter()
     sc.intersect_ids(rays, threads)
     dt = time.perf_counter() - t0
     return {"value": round(len(rays) / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"first {len(rays)} incoherent rays of the workload, oracle restatement, {threads} threads"}
+            "sample": f"the workload's first {len(rays)} incoherent rays, oracle restatement, {threads} threads",
+            "cpu_model": cpu_model()}
 
 
 def render_section(dev, args):
@@ -253,7 +275,7 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         sample = rays[n_coh: n_coh + (1 << args.cpu_sample_log2)]
         print(f"[bench] cpu baseline on {len(sample)} rays, {threads} threads", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(meshes, sample, threads)
+        cpu = cpu_baseline(meshes, sample, threads, args.cpu_repeat)
 
     out = {
         "metric": "incoherent closest-hit Mrays/s @1.3M tris",
