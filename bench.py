@@ -165,6 +165,46 @@ def render_section(dev, args):
             "bvh_build_s": round(info["build_seconds"], 2)}
 
 
+def plain_cpu_render_section(dev, args, threads: int):
+    """The reference's CPU-runnable render (ground quad + three 1,600-face spheres,
+    640x480, 16 spp, depth 5): the reference's own TileRunner on the host's cores
+    beside the GPU's plain and batched paths on the same scene."""
+    import oracle as O
+    from polytope_amd import Scene, render, render_batch
+    from polytope_amd import scenes as S
+    meshes, cam, sky = S.plain_render_scene(1, 640, 480)
+    spp, depth = 16, 5
+    samples = cam.width * cam.height * spp
+    out = {"workload": "ground quad + 3 lat-long spheres (4,802 faces), 640x480, 16 spp, depth 5",
+           "samples": samples}
+    scene = Scene(dev, meshes, sky)
+    render(scene, cam, 1, depth, 1)
+    t = time.perf_counter()
+    render(scene, cam, spp, depth, 1)
+    out["gpu_plain_samples_per_s"] = round(samples / (time.perf_counter() - t), 1)
+    render_batch([(scene, cam, 1, depth, 1)])
+    t = time.perf_counter()
+    render_batch([(scene, cam, spp, depth, 1)])
+    out["gpu_batched_samples_per_s"] = round(samples / (time.perf_counter() - t), 1)
+    scene.close()
+    if O.reference_available() and not args.skip_cpu:
+        with tempfile.TemporaryDirectory() as d:
+            S.write_ptxm(f"{d}/c1.bin", meshes, cam, spp, depth, sky)
+            p = subprocess.run([str(O.REF_DRIVER), "--ptxm", f"{d}/c1.bin", "--time-render", "--threads",
+                                str(threads)], capture_output=True, text=True, timeout=900, cwd=d)
+            line = [ln[ln.index("{\"samples\""):] for ln in p.stdout.splitlines() if "{\"samples\"" in ln]
+            if p.returncode == 0 and line:
+                r = json.loads(line[-1])
+                out["cpu_baseline"] = {"value": round(r["msamples_per_s"] * 1e6, 1), "unit": "samples/s",
+                                       "cores": threads, "kind": "reference",
+                                       "sample": f"the whole frame ({r['seconds']:.2f} s): the reference's "
+                                                 f"TileRunner over PathTraceIntegrator, {threads} threads"}
+                out["gpu_batched_vs_cpu"] = round(out["gpu_batched_samples_per_s"] / out["cpu_baseline"]["value"], 1)
+    print(f"[bench] plain CPU-runnable render: GPU plain {out['gpu_plain_samples_per_s'] / 1e6:.1f}, "
+          f"batched {out['gpu_batched_samples_per_s'] / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
+    return out
+
+
 def serving_section(dev, args):
     """Poisson stream of render jobs through the continuous-batching server."""
     from polytope_amd import Scene, Server
@@ -306,6 +346,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.skip_render:
         out["render"] = render_section(dev, args)
+        out["render_cpu_runnable"] = plain_cpu_render_section(dev, args, args.cpu_threads or min(16, os.cpu_count() or 1))
     if not args.skip_serving:
         out["serving"] = serving_section(dev, args)
     print(json.dumps(out), flush=True)

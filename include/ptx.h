@@ -174,6 +174,15 @@ int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb);
 int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_t n_jobs,
                      float *const *films);
 
+/* Film output (host, no device needed).  film_rgb = W*H*3 averaged floats as
+ * ptx_render returns them.  8-bit conversion follows PNGFilm::Output
+ * (src/cpu/films/PNGFilm.cpp:20-58): min(v, 255) truncated, alpha 255; a NaN
+ * channel becomes 0.  The PNG is 8-bit RGBA; the PFM holds the floats, rows
+ * bottom to top, little-endian. */
+int ptx_film_to_rgba8(const float *film_rgb, uint32_t width, uint32_t height, uint8_t *rgba);
+int ptx_film_write_png(const float *film_rgb, uint32_t width, uint32_t height, const char *path);
+int ptx_film_write_pfm(const float *film_rgb, uint32_t width, uint32_t height, const char *path);
+
 /* Round form of ptx_render_batch and the server (ptx_tuning.render_mode). */
 enum { PTX_RENDER_AUTO = 0, PTX_RENDER_REGEN = 1, PTX_RENDER_WAVEFRONT = 2 };
 

@@ -67,7 +67,7 @@ void usage() {
    std::fprintf(stderr,
       "usage: ref_driver (--ptxm scene.bin | --pbrt scene.pbrt) [--nodes out] [--pairs out]\n"
       "                  [--rays rays.bin --hits out] [--render out] [--threads N]\n"
-      "                  [--time-rays rays.bin --threads N --repeat R]\n");
+      "                  [--time-rays rays.bin --threads N --repeat R] [--time-render --threads N]\n");
    std::exit(2);
 }
 
@@ -75,6 +75,8 @@ void usage() {
 
 int main(int argc, char** argv) {
    std::string ptxm, pbrt, nodes_out, pairs_out, rays_in, hits_out, render_out, time_rays;
+   bool png = false;
+   bool time_render = false;
    unsigned threads = 1, repeat = 1;
    for (int i = 1; i < argc; i++) {
       std::string a = argv[i];
@@ -89,6 +91,8 @@ int main(int argc, char** argv) {
       else if (a == "--time-rays") time_rays = next();
       else if (a == "--threads") threads = (unsigned)std::atoi(next().c_str());
       else if (a == "--repeat") repeat = (unsigned)std::atoi(next().c_str());
+      else if (a == "--time-render") time_render = true;
+      else if (a == "--png") png = true;   // after --render: PNGFilm::Output into the working directory
       else usage();
    }
    Job job;
@@ -191,6 +195,20 @@ int main(int argc, char** argv) {
                   (unsigned long long)checksum.load());
    }
 
+   if (time_render) {
+      // CPU render baseline: the reference's own runner (TileRunner::Run, one
+      // thread per core over its 2x2-pixel tile cursor, src/cpu/runners/TileRunner.cpp:61-117),
+      // spawned as src/cpu/main.cpp:187-211 does.
+      const auto s = std::chrono::steady_clock::now();
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < threads; t++) pool.push_back(runner->Spawn((int)t));
+      for (auto& th : pool) th.join();
+      const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - s).count();
+      const double samples = double(runner->Bounds.x) * runner->Bounds.y * runner->NumSamples;
+      std::printf("{\"samples\": %.0f, \"threads\": %u, \"seconds\": %.6f, \"msamples_per_s\": %.4f}\n",
+                  samples, threads, secs, samples / secs / 1e6);
+   }
+
    if (!render_out.empty()) {
       // All samples of a pixel run in sequence (AbstractRunner::Trace); pixels are
       // split over threads by rows.  Lambertian scenes use the reference's
@@ -213,6 +231,7 @@ int main(int argc, char** argv) {
             p[0] = spd.r; p[1] = spd.g; p[2] = spd.b;
          }
       write_file(render_out, film.data(), film.size() * 4);
+      if (png) runner->Output();
    }
    return 0;
 }

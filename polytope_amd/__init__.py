@@ -237,6 +237,25 @@ def _status_dict(st: _abi.JobStatus) -> dict:
     return {k: getattr(st, k) for k, _ in _abi.JobStatus._fields_}
 
 
+def film_to_rgba8(film: np.ndarray) -> np.ndarray:
+    """(H, W, 3) float film -> (H, W, 4) uint8 as PNGFilm writes it (clamp at 255, truncate)."""
+    film = np.ascontiguousarray(film, np.float32)
+    h, w = film.shape[:2]
+    out = np.zeros((h, w, 4), np.uint8)
+    check(load().ptx_film_to_rgba8(film.ctypes.data, w, h, out.ctypes.data))
+    return out
+
+
+def write_png(film: np.ndarray, path: str) -> None:
+    film = np.ascontiguousarray(film, np.float32)
+    check(load().ptx_film_write_png(film.ctypes.data, film.shape[1], film.shape[0], str(path).encode()))
+
+
+def write_pfm(film: np.ndarray, path: str) -> None:
+    film = np.ascontiguousarray(film, np.float32)
+    check(load().ptx_film_write_pfm(film.ctypes.data, film.shape[1], film.shape[0], str(path).encode()))
+
+
 class Server:
     """Continuous-batching render service (ptx_server_*): jobs are admitted as
     HBM budget and job slots free up and share the GPU round by round."""

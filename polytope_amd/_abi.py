@@ -118,6 +118,9 @@ SIGNATURES = {
     "ptx_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "ptx_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "ptx_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Tuning)]),
+    "ptx_film_to_rgba8": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "ptx_film_write_png": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_char_p]),
+    "ptx_film_write_pfm": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_char_p]),
     "ptx_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "ptx_server_destroy": (None, [C.c_void_p]),
     "ptx_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),

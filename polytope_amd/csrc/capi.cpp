@@ -51,8 +51,8 @@ uint32_t leaf_lanes_of(const ptx_device *dev, bool render) {
    return render ? kDefaultRenderLeafLanes : kDefaultLeafLanes;
 }
 
-uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds) {
-   int per_cu = closest_hit_blocks_per_cu(lds);
+uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds, bool extra) {
+   int per_cu = closest_hit_blocks_per_cu(lds, extra);
    if (dev->tuning.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)dev->tuning.blocks_per_cu);
    return (uint32_t)(dev->num_cus * per_cu);
 }
@@ -67,7 +67,7 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
                     ptx_hit_frame *d_frames, hipStream_t stream) {
    ptx_device *dev = s->dev;
    const uint32_t lds = lds_depth_of(dev);
-   uint32_t grid = closest_hit_grid(dev, lds);
+   uint32_t grid = closest_hit_grid(dev, lds, d_counters != nullptr || d_frames != nullptr);
    const uint32_t needed_blocks = (n + 255) / 256;
    grid = std::max(1u, std::min(grid, needed_blocks));
    void *scratch = nullptr;

@@ -12,7 +12,8 @@ int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_
                        ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
                        uint32_t grid, uint32_t lds_depth, uint32_t refill_min, uint32_t leaf_min,
                        hipStream_t stream);
-int closest_hit_blocks_per_cu(uint32_t lds_depth);
+// Resident blocks per CU of the closest-hit kernel; extra = counters or frames requested.
+int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra);
 
 }  // namespace ptx
 

@@ -47,7 +47,7 @@ uint32_t lds_depth_of(const ptx_device *dev);
 constexpr uint32_t kDefaultLeafLanes = 1;
 constexpr uint32_t kDefaultRenderLeafLanes = 16;
 uint32_t leaf_lanes_of(const ptx_device *dev, bool render = false);
-uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds);
+uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds, bool extra);
 size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds);
 int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
                     ptx_hit_frame *d_frames, hipStream_t stream);

@@ -2,6 +2,8 @@
 // persistent traversal of traverse.h over an array of rays, one scene.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "traverse.h"
 
 #pragma clang fp contract(off)
@@ -9,6 +11,10 @@
 namespace ptx {
 namespace dev {
 
+// kExtra: the caller may want the per-ray counters or frames; without it the
+// kernel writes hits only, and the counter and face-test bookkeeping of the
+// traversal is compiled out.
+template <bool kExtra>
 __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, uint32_t id, ptx_hit *hits,
                                              ptx_ray_counters *counters, ptx_hit_frame *frames) {
    ptx_hit h;
@@ -21,6 +27,7 @@ __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, 
       h.t = kInf; h.u = h.v = h.w = 0.f;
    }
    hits[id] = h;
+   if (!kExtra) return;
    if (counters) counters[id] = ptx_ray_counters{s.bb_hits, s.bb_misses, s.tris};
    if (frames) {
       ptx_hit_frame fr;
@@ -38,6 +45,7 @@ __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, 
    }
 }
 
+template <bool kExtra>
 struct RayQuery {
    struct Ctx {};
    DevScene sc;
@@ -63,16 +71,16 @@ struct RayQuery {
    __device__ __forceinline__ const float *tris(const Ctx &) const { return sc.tris; }
    __device__ __forceinline__ bool debug_pixel(const Ctx &) const { return false; }
    __device__ __forceinline__ void finish(uint32_t id, const Ctx &, const Trav &s) const {
-      write_result(s, sc, id, hits, counters, frames);
+      write_result<kExtra>(s, sc, id, hits, counters, frames);
    }
 };
 
-template <int kLds>
+template <int kLds, bool kExtra>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n, ptx_hit *__restrict__ hits,
               ptx_ray_counters *counters, ptx_hit_frame *frames, uint32_t *work, uint4 *spill, uint32_t refill_min,
               uint32_t leaf_min) {
-   const RayQuery q{sc, rays, hits, counters, frames};
+   const RayQuery<kExtra> q{sc, rays, hits, counters, frames};
    persistent_trace<kLds>(q, n, work, spill, refill_min, leaf_min, sc.max_stack);
 }
 
@@ -85,6 +93,31 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 
 // Host launcher.  work: >= 8 u32 of device memory; spill sized by the caller
 // for (max_stack - lds) * grid * 256 entries.
+namespace {
+
+template <int kLds>
+void launch_lds(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
+                ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill, uint32_t grid, uint32_t refill_min,
+                uint32_t leaf_min, hipStream_t stream) {
+   if (d_counters || d_frames)
+      hipLaunchKernelGGL((dev::k_closest_hit<kLds, true>), dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
+                         d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+   else
+      hipLaunchKernelGGL((dev::k_closest_hit<kLds, false>), dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
+                         d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+}
+
+template <int kLds>
+int blocks_lds(bool extra) {
+   int b = 0;
+   const hipError_t e =
+         extra ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_closest_hit<kLds, true>, 256, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_closest_hit<kLds, false>, 256, 0);
+   return e == hipSuccess && b > 0 ? b : 1;
+}
+
+}  // namespace
+
 int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits,
                        ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
                        uint32_t grid, uint32_t lds_depth, uint32_t refill_min, uint32_t leaf_min,
@@ -93,35 +126,31 @@ int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_
    if (n == 0) return 0;
    switch (lds_depth) {
       case 8:
-         hipLaunchKernelGGL(dev::k_closest_hit<8>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+         launch_lds<8>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min, stream);
          break;
       case 16:
-         hipLaunchKernelGGL(dev::k_closest_hit<16>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+         launch_lds<16>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min,
+                        stream);
          break;
       case 24:
-         hipLaunchKernelGGL(dev::k_closest_hit<24>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+         launch_lds<24>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min,
+                        stream);
          break;
       default:
-         hipLaunchKernelGGL(dev::k_closest_hit<12>, dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                            d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+         launch_lds<12>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min,
+                        stream);
          break;
    }
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int closest_hit_blocks_per_cu(uint32_t lds_depth) {
-   int blocks = 0;
-   hipError_t e;
+int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra) {
    switch (lds_depth) {
-      case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::k_closest_hit<8>, 256, 0); break;
-      case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::k_closest_hit<16>, 256, 0); break;
-      case 24: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::k_closest_hit<24>, 256, 0); break;
-      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::k_closest_hit<12>, 256, 0); break;
+      case 8: return blocks_lds<8>(extra);
+      case 16: return blocks_lds<16>(extra);
+      case 24: return blocks_lds<24>(extra);
+      default: return blocks_lds<12>(extra);
    }
-   return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
 }  // namespace ptx
