@@ -75,6 +75,15 @@ typedef struct ptx_bvh_node {
    uint16_t flags;         /* split axis                                           */
 } ptx_bvh_node;
 
+/* Point light for next-event estimation (PTX_JOB_NEE).  The reference parses
+ * LightSource "point" but never uses it (pbrt_parser.cpp:1348), so scenes
+ * rendered for parity carry none; intensity is in the same 0-255 units as
+ * emission and the skybox. */
+typedef struct ptx_point_light {
+   float position[3];
+   float intensity[3];
+} ptx_point_light;
+
 typedef struct ptx_scene_desc {
    const ptx_mesh_desc *meshes;
    uint32_t num_meshes;
@@ -88,6 +97,8 @@ typedef struct ptx_scene_desc {
    int32_t has_skybox;     /* Scene::Skybox != nullptr (ColorSkybox)  */
    float skybox[3];
    uint32_t build_threads; /* 0 = all host cores */
+   const ptx_point_light *lights;   /* optional, for jobs with PTX_JOB_NEE */
+   uint32_t num_lights;
 } ptx_scene_desc;
 
 typedef struct ptx_scene_info {
@@ -109,9 +120,22 @@ typedef struct ptx_camera_desc {
 /* One render: CenterSampler, PathTraceIntegrator{MaxDepth}, BoxFilter film.
  * Lambert bounces draw u0, u1 from Philox4x32-10 with key (seed, 0x706F6C79)
  * and counter (y*W + x, sample, bounce, 0): u = (r >> 8) * 2^-24. */
+/* ptx_job_desc.flags */
+enum {
+   /* Next-event estimation (no reference counterpart; the reference's shadow
+    * connection is commented out, PathTraceIntegrator.cpp:97-130): at each
+    * Lambert vertex, for each point light in order, l = normalize(light - P),
+    * cos = n . l; when cos > 0 and a shadow ray (P, l, t < |light - P|) hits
+    * nothing, direct += (throughput * Kd) * (I * (cos / (pi * |light - P|^2))).
+    * The sample is its terminal radiance plus direct.  Off: the reference's
+    * integrator exactly. */
+   PTX_JOB_NEE = 1
+};
+
 typedef struct ptx_job_desc {
    ptx_camera_desc camera;
    uint32_t spp, max_depth, seed;
+   uint32_t flags;         /* PTX_JOB_* */
 } ptx_job_desc;
 
 /* Closest hit of one ray.  face == 0xFFFFFFFF: miss (mesh also 0xFFFFFFFF). */

@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
         L = C.CDLL(str(LIB))
         L.orc_scene_create.restype = C.c_void_p
         L.orc_scene_create.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+        L.orc_scene_set_lights.restype = None
+        L.orc_scene_set_lights.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
         L.orc_scene_create_prebuilt.restype = C.c_void_p
         L.orc_scene_create_prebuilt.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_uint32,
                                                 C.c_void_p, C.c_uint32]
@@ -80,15 +82,16 @@ def lib() -> C.CDLL:
 
 class OrcJob(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("max_depth", C.c_uint32), ("seed", C.c_uint32)]
+                ("max_depth", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class OracleScene:
     """The oracle's scene: world vertices, the restated reference BVH."""
 
-    def __init__(self, meshes, skybox: Optional[Sequence[float]] = None, bvh=None):
+    def __init__(self, meshes, skybox: Optional[Sequence[float]] = None, bvh=None, lights=None):
         """bvh: optional (nodes, pairs) in the reference compact layout to use
-        instead of building the restated tree."""
+        instead of building the restated tree; lights: optional point lights
+        [(position xyz, intensity rgb), ...] for next-event estimation."""
         from polytope_amd import mesh_descs
         arr, keep = mesh_descs(meshes)
         sky = (C.c_float * 3)(*(skybox if skybox is not None else (0.0, 0.0, 0.0)))
@@ -102,6 +105,9 @@ class OracleScene:
                                                      1 if skybox is not None else 0, C.cast(sky, C.c_void_p),
                                                      nodes.ctypes.data, len(nodes), pairs.ctypes.data, len(pairs))
         self._keep = (arr, keep)
+        if lights:
+            flat = np.ascontiguousarray([[*p, *i] for p, i in lights], np.float32)
+            lib().orc_scene_set_lights(self.h, flat.ctypes.data, len(flat))
 
     def bvh(self) -> Tuple[np.ndarray, np.ndarray]:
         n = lib().orc_num_nodes(self.h)
@@ -123,10 +129,11 @@ class OracleScene:
         s = lib().orc_intersect_ids(self.h, rays.ctypes.data, rays.shape[0], ids.ctypes.data, threads)
         return ids, int(s)
 
-    def render(self, cam, spp: int, max_depth: int, seed: int, threads: int = 0, row_step: int = 1) -> np.ndarray:
+    def render(self, cam, spp: int, max_depth: int, seed: int, threads: int = 0, row_step: int = 1,
+               nee: bool = False) -> np.ndarray:
         from polytope_amd import camera_desc
         cd = camera_desc(cam)
-        job = OrcJob(cam.width, cam.height, spp, max_depth, seed)
+        job = OrcJob(cam.width, cam.height, spp, max_depth, seed, 1 if nee else 0)
         film = np.zeros((cam.height, cam.width, 3), np.float32)
         lib().orc_render(self.h, C.byref(cd), C.byref(job), film.ctypes.data, threads or os.cpu_count(), row_step)
         return film

@@ -217,12 +217,18 @@ struct BuildNode {
    size_t lo = 0, hi = 0;            // leaf index range into the shared index array
 };
 
+struct PointLight {
+   V3 p;
+   float I[3];
+};
+
 struct Scene {
    std::vector<Mesh> meshes;
    std::vector<CompactNode> nodes;
    std::vector<Pair> pairs;
    bool has_sky = false;
    float sky[3] = {0, 0, 0};
+   std::vector<PointLight> lights;     // used only by next-event estimation
 };
 
 // bvh::bound (src/cpu/acceleration/bvh.cpp:36-395).  The reference copies each
@@ -576,6 +582,39 @@ static void intersect(const Scene& s, Ray& ray, Isect& is) {
    if (is.hits) hit_frame(s, ray, is);
 }
 
+// Shadow ray for next-event estimation (no reference counterpart: the
+// reference's shadow test is commented out, PathTraceIntegrator.cpp:97-130).
+// Closest-hit traversal bounded by tmax (ray.min_t starts at tmax, so boxes and
+// faces beyond it are culled exactly as a closer hit would cull them) that
+// stops at the first accepted face.  No debug-pixel quirk on shadow rays.
+static bool occluded(const Scene& s, const V3& o, const V3& d, float tmax) {
+   Ray ray{o, d, tmax};
+   Isect is;
+   is.px = -1; is.py = -1;
+   const V3 inv = {1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z};
+   const bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+   thread_local std::vector<uint32_t> stack;
+   stack.clear();
+   stack.push_back(0);
+   while (!stack.empty()) {
+      const uint32_t ni = stack.back();
+      stack.pop_back();
+      const CompactNode& node = s.nodes[ni];
+      if (!box_hits(node.bb, ray.o, inv, ray.min_t)) continue;
+      if (node.num_faces > 0) {
+         for (uint32_t i = 0; i < node.num_faces; i++) {
+            const Pair& p = s.pairs[node.offset + i];
+            test_face(s.meshes[p.first], p.first, p.second, ray, is);
+            if (is.hits) return true;
+         }
+         continue;
+      }
+      if (neg[node.flags]) { stack.push_back(ni + 1); stack.push_back(ni + node.offset); }
+      else { stack.push_back(ni + node.offset); stack.push_back(ni + 1); }
+   }
+   return false;
+}
+
 // ----------------------------------------------------------------------------- RNG, sincos
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
@@ -638,6 +677,7 @@ struct CameraIn {                 // mirrors ptx_camera_desc
 struct JobIn {                    // mirrors ptx_job_desc
    uint32_t width, height, spp, max_depth;
    uint32_t seed;
+   uint32_t flags;                // bit 0: next-event estimation to the point lights
 };
 
 struct Camera {
@@ -661,13 +701,42 @@ struct Camera {
 struct Spd { float r, g, b; };
 
 // PathTraceIntegrator::GetSample (src/cpu/integrators/PathTraceIntegrator.cpp:10-135)
+// Next-event estimation at a Lambert vertex (opt-in; no reference
+// counterpart): for each point light in order, l = normalize(light - P) with
+// d2 = |light - P|^2, cos = n . l; if cos > 0 and the shadow ray (P, l, t < d)
+// is unoccluded, direct += (src * Kd) * (I * (cos / (pi * d2))).  P is the
+// offset hit Location, n the shading normal facing the incoming ray, src the
+// throughput before this vertex's reflectance.
+static void add_direct(const Scene& s, const Isect& is, const float refl[3], float sr, float sg, float sb, Spd& direct) {
+   for (const PointLight& L : s.lights) {
+      const float wx = L.p.x - is.loc.x, wy = L.p.y - is.loc.y, wz = L.p.z - is.loc.z;
+      const float d2 = wx * wx + wy * wy + wz * wz;
+      const float d = std::sqrt(d2);
+      const float inv = 1.0f / d;
+      const V3 l = {wx * inv, wy * inv, wz * inv};
+      const float cosv = is.n.x * l.x + is.n.y * l.y + is.n.z * l.z;
+      if (!(cosv > 0.f)) continue;
+      if (occluded(s, is.loc, l, d)) continue;
+      const float g = cosv / (kPI * d2);
+      direct.r = direct.r + (sr * refl[0]) * (L.I[0] * g);
+      direct.g = direct.g + (sg * refl[1]) * (L.I[1] * g);
+      direct.b = direct.b + (sb * refl[2]) * (L.I[2] * g);
+   }
+}
+
 static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t pixel_index,
-                      uint32_t sample_index, uint32_t max_depth, uint32_t seed) {
+                      uint32_t sample_index, uint32_t max_depth, uint32_t seed, bool nee) {
    Ray cur = primary;
    uint32_t bounces = 0;
    float sr = 1.f, sg = 1.f, sb = 1.f;
+   Spd direct = {0, 0, 0};
+   // terminal radiance, plus the next-event sum when it is on
+   auto done = [&](Spd t) -> Spd {
+      if (nee) { t.r = t.r + direct.r; t.g = t.g + direct.g; t.b = t.b + direct.b; }
+      return t;
+   };
    while (true) {
-      if (sr < 0.0039f && sg < 0.0039f && sb < 0.0039f) return {0, 0, 0};
+      if (sr < 0.0039f && sg < 0.0039f && sb < 0.0039f) return done({0, 0, 0});
       Isect is;
       is.px = x; is.py = y;
       cur.min_t = kInf;
@@ -676,11 +745,12 @@ static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t
          Spd spd = {0, 0, 0};
          if (s.has_sky) spd = {s.sky[0] * sr, s.sky[1] * sg, s.sky[2] * sb};
          spd.r += 0.f; spd.g += 0.f; spd.b += 0.f;
-         return spd;
+         return done(spd);
       }
       const Mesh& m = s.meshes[is.mesh];
-      if (m.light) return {(m.L[0] + 0.f) * sr, (m.L[1] + 0.f) * sg, (m.L[2] + 0.f) * sb};
-      if (bounces >= max_depth) return {0, 0, 0};
+      if (m.light) return done({(m.L[0] + 0.f) * sr, (m.L[1] + 0.f) * sg, (m.L[2] + 0.f) * sb});
+      if (bounces >= max_depth) return done({0, 0, 0});
+      if (nee && m.brdf != 2) add_direct(s, is, m.refl, sr, sg, sb, direct);
       const uint32_t bounce_index = bounces;
       bounces++;
       // Intersection::WorldToLocal (Intersection.cpp:10-14); bent normal == geo normal
@@ -796,6 +866,14 @@ orc::Scene* orc_scene_create_prebuilt(const orc::MeshIn* meshes, uint32_t n_mesh
 }
 
 void orc_scene_destroy(orc::Scene* s) { delete s; }
+
+// Point lights for next-event estimation: 6 floats each (position, intensity).
+void orc_scene_set_lights(orc::Scene* s, const float* lights, uint32_t n) {
+   s->lights.resize(n);
+   for (uint32_t i = 0; i < n; i++)
+      s->lights[i] = {{lights[6 * i], lights[6 * i + 1], lights[6 * i + 2]},
+                      {lights[6 * i + 3], lights[6 * i + 4], lights[6 * i + 5]}};
+}
 uint32_t orc_num_nodes(const orc::Scene* s) { return (uint32_t)s->nodes.size(); }
 uint32_t orc_num_pairs(const orc::Scene* s) { return (uint32_t)s->pairs.size(); }
 void orc_copy_bvh(const orc::Scene* s, void* nodes, uint32_t* pairs) {
@@ -885,7 +963,8 @@ void orc_render(const orc::Scene* s, const orc::CameraIn* cam, const orc::JobIn*
                float ar = 0, ag = 0, ab = 0;
                const uint32_t pix = y * W + x;
                for (uint32_t sidx = 0; sidx < job->spp; sidx++) {
-                  orc::Spd smp = orc::get_sample(*s, primary, (int)x, (int)y, pix, sidx, job->max_depth, job->seed);
+                  orc::Spd smp = orc::get_sample(*s, primary, (int)x, (int)y, pix, sidx, job->max_depth, job->seed,
+                                                 (job->flags & 1u) != 0);
                   if (smp.r > 255) smp.r = 255;
                   if (smp.g > 255) smp.g = 255;
                   if (smp.b > 255) smp.b = 255;

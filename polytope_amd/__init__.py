@@ -69,8 +69,10 @@ def camera_desc(cam: Camera) -> _abi.CameraDesc:
     return c
 
 
-def job_desc(cam: Camera, spp: int, max_depth: int, seed: int) -> _abi.JobDesc:
+def job_desc(cam: Camera, spp: int, max_depth: int, seed: int, nee: bool = False) -> _abi.JobDesc:
+    """nee: next-event estimation to the scene's point lights (PTX_JOB_NEE)."""
     j = _abi.JobDesc()
+    j.flags = _abi.JOB_NEE if nee else 0
     j.camera = camera_desc(cam)
     j.spp = spp
     j.max_depth = max_depth
@@ -138,7 +140,10 @@ class Scene:
     builds it on the host."""
 
     def __init__(self, device: Device, meshes: Sequence[MeshData], skybox: Optional[Sequence[float]] = None,
-                 bvh: Optional[Tuple[np.ndarray, np.ndarray]] = None, build_threads: int = 0):
+                 bvh: Optional[Tuple[np.ndarray, np.ndarray]] = None, build_threads: int = 0,
+                 lights: Optional[Sequence[Tuple[Sequence[float], Sequence[float]]]] = None):
+        """lights: point lights [(position xyz, intensity rgb), ...] for jobs
+        rendered with next-event estimation."""
         lib = load()
         self.device = device
         self.meshes = list(meshes)
@@ -158,6 +163,11 @@ class Scene:
         if skybox is not None:
             desc.skybox[:] = [float(x) for x in skybox]
         desc.build_threads = build_threads
+        if lights:
+            flat = np.ascontiguousarray([[*p, *i] for p, i in lights], np.float32).reshape(-1)
+            keep.append(flat)
+            desc.lights = flat.ctypes.data_as(C.POINTER(C.c_float))
+            desc.num_lights = len(lights)
         h = C.c_void_p()
         check(lib.ptx_scene_create(device.handle, C.byref(desc), C.byref(h)))
         self.handle = h
@@ -204,21 +214,21 @@ class Scene:
             pass
 
 
-def render(scene: Scene, cam: Camera, spp: int, max_depth: int, seed: int) -> np.ndarray:
+def render(scene: Scene, cam: Camera, spp: int, max_depth: int, seed: int, nee: bool = False) -> np.ndarray:
     """Plain GPU render of one job (ptx_render): one launch per sample, one thread
     per pixel path.  Returns the (H, W, 3) BoxFilter film (0-255 units)."""
     film = np.zeros((cam.height, cam.width, 3), np.float32)
-    job = job_desc(cam, spp, max_depth, seed)
+    job = job_desc(cam, spp, max_depth, seed, nee)
     check(load().ptx_render(scene.handle, C.byref(job), film.ctypes.data))
     return film
 
 
-def render_batch(jobs: Sequence[Tuple[Scene, Camera, int, int, int]]) -> List[np.ndarray]:
-    """Batched wavefront render (ptx_render_batch) of (scene, camera, spp,
-    max_depth, seed) jobs in one call; returns one film per job."""
+def render_batch(jobs: Sequence[tuple]) -> List[np.ndarray]:
+    """Batched render (ptx_render_batch) of (scene, camera, spp, max_depth, seed[,
+    nee]) jobs in one call; returns one film per job."""
     n = len(jobs)
     handles = (C.c_void_p * n)(*[j[0].handle.value for j in jobs])
-    descs = (_abi.JobDesc * n)(*[job_desc(j[1], j[2], j[3], j[4]) for j in jobs])
+    descs = (_abi.JobDesc * n)(*[job_desc(j[1], j[2], j[3], j[4], len(j) > 5 and bool(j[5])) for j in jobs])
     films = [np.zeros((j[1].height, j[1].width, 3), np.float32) for j in jobs]
     ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in films])
     check(load().ptx_render_batch(handles, descs, n, ptrs))
@@ -269,8 +279,8 @@ class Server:
         self.device = device
         self._dims = {}
 
-    def submit(self, scene: Scene, cam: Camera, spp: int, max_depth: int, seed: int) -> int:
-        job = job_desc(cam, spp, max_depth, seed)
+    def submit(self, scene: Scene, cam: Camera, spp: int, max_depth: int, seed: int, nee: bool = False) -> int:
+        job = job_desc(cam, spp, max_depth, seed, nee)
         jid = C.c_uint64()
         check(load().ptx_server_submit(self.handle, scene.handle, C.byref(job), C.byref(jid)))
         self._dims[jid.value] = (cam.height, cam.width)

@@ -52,7 +52,7 @@ class SceneDesc(C.Structure):
                 ("nodes", C.POINTER(BvhNode)), ("num_nodes", C.c_uint32),
                 ("leaf_pairs", _up), ("num_pairs", C.c_uint32),
                 ("has_skybox", C.c_int32), ("skybox", C.c_float * 3),
-                ("build_threads", C.c_uint32)]
+                ("build_threads", C.c_uint32), ("lights", C.POINTER(C.c_float)), ("num_lights", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -69,7 +69,11 @@ class CameraDesc(C.Structure):
 
 
 class JobDesc(C.Structure):
-    _fields_ = [("camera", CameraDesc), ("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint32)]
+    _fields_ = [("camera", CameraDesc), ("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+JOB_NEE = 1
 
 
 class Tuning(C.Structure):

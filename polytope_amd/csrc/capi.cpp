@@ -135,6 +135,8 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    if (!dev || !desc || !out) return set_error(PTX_ERR_INVALID, "NULL argument");
    *out = nullptr;
    if (desc->num_meshes == 0 || !desc->meshes) return set_error(PTX_ERR_INVALID, "scene has no meshes");
+   if (desc->num_lights && !desc->lights) return set_error(PTX_ERR_INVALID, "num_lights > 0 with NULL lights");
+   if (desc->num_lights > 64) return set_error(PTX_ERR_INVALID, "at most 64 point lights");
    for (uint32_t i = 0; i < desc->num_meshes; i++) {
       const ptx_mesh_desc &m = desc->meshes[i];
       if (!m.x || !m.y || !m.z || !m.fv0 || !m.fv1 || !m.fv2)
@@ -171,6 +173,15 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    if ((rc = upload(s.get(), s->bvh.pairs, &s->d.pairs)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.meshes, &s->d.meshes)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.normals, &s->d.normals)) != PTX_OK) return rc;
+   if (desc->num_lights) {
+      std::vector<float> lights(6 * (size_t)desc->num_lights);
+      for (uint32_t i = 0; i < desc->num_lights; i++) {
+         std::copy(desc->lights[i].position, desc->lights[i].position + 3, &lights[6 * i]);
+         std::copy(desc->lights[i].intensity, desc->lights[i].intensity + 3, &lights[6 * i + 3]);
+      }
+      if ((rc = upload(s.get(), lights, &s->d.lights)) != PTX_OK) return rc;
+   }
+   s->d.num_lights = desc->num_lights;
    std::copy(img.root_box, img.root_box + 6, s->d.root_box);
    s->d.root_ref = img.root_ref;
    s->d.root_count = img.root_count;

@@ -20,7 +20,7 @@ int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra);
 namespace ptx {
 
 uint32_t render_lds_depth();
-int render_blocks_per_cu(int which);   // 0 extend, 1 primary, 2 plain megakernel
+int render_blocks_per_cu(int which);   // 0 extend, 1 primary, 2 plain megakernel, 3 plain with next-event estimation
 void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t leaf_min, uint32_t grid,
                     hipStream_t st);
 void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs,
@@ -36,12 +36,14 @@ void launch_finalize(float *film, uint32_t npix, uint32_t spp, hipStream_t st);
 void launch_zero_floats(float *a, size_t n, hipStream_t st);
 int paths_blocks_per_cu();
 // Path-regeneration mode: one launch per round; rounds limited to
-// paths_max_chunks() chunks and paths_max_scenes() scenes.
+// paths_max_chunks() chunks and paths_max_scenes() scenes.  nee_scratch (48
+// bytes per resident lane, grid * 256 lanes) selects the kernel variant with
+// next-event estimation for the jobs that ask for it; null = without.
 uint32_t paths_max_chunks();
 uint32_t paths_max_scenes();
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-                  uint32_t leaf_min, uint32_t grid, hipStream_t st);
+                  uint32_t leaf_min, float4 *nee_scratch, uint32_t grid, hipStream_t st);
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st);
 
 }  // namespace ptx

@@ -60,8 +60,11 @@ struct DevScene {
    uint32_t max_stack;       // deepest deferred-stack use + 1 (tree depth bound)
    int32_t has_sky;
    float sky[3];
+   const float *lights;      // point lights, 6 floats each (position, intensity), or null
+   uint32_t num_lights;
+   uint32_t pad;
 };
-static_assert(sizeof(DevScene) == 96, "DevScene layout");
+static_assert(sizeof(DevScene) == 112, "DevScene layout");
 
 // Per-job constants of a render, device-resident in the job table.
 struct JobDev {
@@ -69,6 +72,7 @@ struct JobDev {
    float oow, ooh, aspect, tanf2;
    uint32_t width, height, spp, max_depth;
    uint32_t seed, scene;     // RNG seed; index into the scene table
+   uint32_t flags, pad;      // PTX_JOB_*
    float *film;              // W*H*3 running sums (BoxFilter order), then the average
    uint32_t *primary;        // W*H primary-hit records (16 words each)
 };

@@ -170,6 +170,41 @@ __device__ __forceinline__ bool src_is_zero(const float src[3]) {        // spec
    return src[0] < 0.0039f && src[1] < 0.0039f && src[2] < 0.0039f;
 }
 
+// ---------------------------------------------------------------- next-event estimation
+
+// One point light seen from P with shading normal n: direction l, distance d and
+// the geometry factor g = cos / (pi * d^2); returns false when cos <= 0.
+__device__ __forceinline__ bool light_setup(const float *light, float px, float py, float pz, float nx, float ny,
+                                            float nz, float l[3], float &d, float &g) {
+   const float wx = light[0] - px, wy = light[1] - py, wz = light[2] - pz;
+   const float d2 = wx * wx + wy * wy + wz * wz;
+   d = __builtin_sqrtf(d2);
+   const float inv = 1.0f / d;
+   l[0] = wx * inv; l[1] = wy * inv; l[2] = wz * inv;
+   const float cosv = nx * l[0] + ny * l[1] + nz * l[2];
+   if (!(cosv > 0.f)) return false;
+   g = cosv / (kPI * d2);
+   return true;
+}
+
+// Shadow ray: closest-hit traversal bounded by tmax that stops at the first
+// accepted face.  No debug-pixel quirk on shadow rays.
+template <int kLds>
+__device__ __forceinline__ bool occluded(const DevScene &S, const Stack<kLds> &st, float ox, float oy, float oz,
+                                         const float l[3], float tmax) {
+   Trav s;
+   make_ray(s.r, ox, oy, oz, l[0], l[1], l[2]);
+   begin(s, S.root_box, S.root_ref, S.root_count, tmax);
+   while (s.cur != kNone) {
+      Fetched f;
+      fetch(s, S.nodes, S.tris, f);
+      const bool more = advance<kLds>(s, f, st, false);
+      if (s.hit != kNone) return true;
+      if (!more) break;
+   }
+   return false;
+}
+
 __device__ __forceinline__ Frame frame_of(const DevScene &S, uint32_t tri, const float d[3], float u, float v,
                                           float w, uint32_t after, uint32_t mesh) {
    Ray r;
@@ -521,7 +556,7 @@ __global__ void k_zero_floats(float *a, size_t n) {
 // ---------------------------------------------------------------- plain path
 
 // One sample of every pixel: one thread per pixel path, all bounces inline.
-template <int kLds>
+template <int kLds, bool kNee>
 __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sample, uint4 *spill) {
    __shared__ uint32_t lds_ref[kLds * 256];
    __shared__ float lds_near[kLds * 256];
@@ -536,6 +571,8 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
       camera_ray(J, x, y, o, d);
       float src[3] = {1.f, 1.f, 1.f};
       Radiance L = {0.f, 0.f, 0.f};
+      const bool nee = kNee && (J.flags & PTX_JOB_NEE) != 0;
+      float direct[3] = {0.f, 0.f, 0.f};
       for (uint32_t b = 0;; b++) {
          if (src_is_zero(src)) break;
          Trav s;
@@ -559,11 +596,27 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
          }
          if (b >= J.max_depth) break;
          const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
+         if (nee && M.brdf != PTX_BRDF_MIRROR) {
+            for (uint32_t j = 0; j < S.num_lights; j++) {
+               const float *light = S.lights + 6 * (size_t)j;
+               float l[3], dist, g;
+               if (!light_setup(light, fr.lx, fr.ly, fr.lz, fr.nx, fr.ny, fr.nz, l, dist, g)) continue;
+               if (occluded<kLds>(S, st, fr.lx, fr.ly, fr.lz, l, dist)) continue;
+               direct[0] = direct[0] + (src[0] * M.refl[0]) * (light[3] * g);
+               direct[1] = direct[1] + (src[1] * M.refl[1]) * (light[4] * g);
+               direct[2] = direct[2] + (src[2] * M.refl[2]) * (light[5] * g);
+            }
+         }
          float lin[3], wout[3];
          world_to_local(fr, d, lin);
          bounce(M, fr, lin, p, sample, b, J.seed, wout, src);
          o[0] = fr.lx; o[1] = fr.ly; o[2] = fr.lz;
          d[0] = wout[0]; d[1] = wout[1]; d[2] = wout[2];
+      }
+      if (nee) {
+         L.r = L.r + direct[0];
+         L.g = L.g + direct[1];
+         L.b = L.b + direct[2];
       }
       float *film = J.film + 3 * (size_t)p;
       float sr = L.r, sg = L.g, sb = L.b;
@@ -590,20 +643,26 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
 // BRDF sample) cost several traversal steps and run for the whole wave when any
 // lane takes them, so a wave shades only when at least shade_min of its lanes
 // wait for it, or when none is tracing; until then those lanes hold.
-enum : uint32_t { kPIdle = 0, kPPrim = 1, kPTrace = 2, kPShade1 = 3, kPShade2 = 4, kPShade3 = 5, kPHold = 6 };
-// Lane word: bits 0-2 state, bit 3 the reference's debug pixel, bits 4-9 the
-// path's chunk in the round, bits 10-31 its bounce index.
-constexpr uint32_t kPModeMask = 7u, kPDbg = 8u, kPChunkShift = 4, kPBounceShift = 10;
+enum : uint32_t {
+   kPIdle = 0, kPPrim = 1, kPTrace = 2, kPShade1 = 3, kPShade2 = 4, kPShade3 = 5,
+   kPLight = 6,    // next-event estimation: add the last light's term, set up the next shadow ray
+   kPShadow = 7,   // next-event estimation: shadow-ray traversal
+   kPHold = 8      // this iteration only: a lane that waits for its wave to gather
+};
+// Lane word: bits 0-3 state, bit 4 the reference's debug pixel, bits 5-10 the
+// path's chunk in the round, bits 11-30 its bounce index, bit 31 set when the
+// path's job uses next-event estimation.
+constexpr uint32_t kPModeMask = 15u, kPDbg = 16u, kPChunkShift = 5, kPBounceShift = 11, kPNee = 1u << 31;
 constexpr uint32_t kMaxRoundScenes = 64;
 static_assert(kMaxRoundChunks <= 64, "chunk index has 6 bits in the lane word");
 
 __device__ __forceinline__ uint32_t lw_chunk(uint32_t w) { return (w >> kPChunkShift) & 63u; }
-__device__ __forceinline__ uint32_t lw_bounce(uint32_t w) { return w >> kPBounceShift; }
+__device__ __forceinline__ uint32_t lw_bounce(uint32_t w) { return (w & ~kPNee) >> kPBounceShift; }
 __device__ __forceinline__ uint32_t lw_with_state(uint32_t w, uint32_t state) { return (w & ~kPModeMask) | state; }
 
 struct LdsChunk {
    uint32_t base, npix, s0, width;
-   uint32_t seed, max_depth, scene, pad;
+   uint32_t seed, max_depth, scene, flags;
    const uint32_t *primary;
    const void *pad2;
 };
@@ -617,21 +676,39 @@ __device__ __forceinline__ uint32_t lds_find_chunk(const LdsChunk *c, uint32_t n
    return lo;
 }
 
+// Terminal radiance of a path plus its next-event sum (paths of next-event jobs).
+template <bool kNee>
+__device__ __forceinline__ Radiance fin(Radiance R, const float direct[3], uint32_t lane_word) {
+   if (kNee && (lane_word & kPNee)) {
+      R.r = R.r + direct[0];
+      R.g = R.g + direct[1];
+      R.b = R.b + direct[2];
+   }
+   return R;
+}
+
 // Starts the traversal of ray s.r in scene S; returns the next state, or kPIdle
 // after writing the miss result when the root box is missed.
+template <bool kNee>
 __device__ __forceinline__ uint32_t p_root(const DevScene &S, Trav &s, float *res, uint32_t slot,
-                                           const float src[3]) {
+                                           const float src[3], const float direct[3], uint32_t lane_word) {
    begin(s, S.root_box, S.root_ref, S.root_count);
    if (s.cur != kNone) return kPTrace;
-   write_res(res, slot, miss_radiance(S, src));
+   write_res(res, slot, fin<kNee>(miss_radiance(S, src), direct, lane_word));
    return kPIdle;
 }
 
-template <int kLds>
+// Per-lane record of a next-event vertex (3 x 16 bytes in nee_scratch): the
+// vertex P, the next ray's direction, throughput x Kd, the shading normal.
+struct NeeRecord {
+   float4 a, b, c;   // a = P.xyz, wout.x; b = wout.yz, k.xy; c = k.z, n.xyz
+};
+
+template <int kLds, bool kNee>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
         uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-        uint32_t leaf_min) {
+        uint32_t leaf_min, float4 *nee_scratch) {
    __shared__ uint32_t lds_ref[kLds * 256];
    __shared__ float lds_near[kLds * 256];
    __shared__ uint16_t lds_cnt[kLds * 256];
@@ -642,15 +719,16 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
       const JobDev &J = jobs[c.job];
       LdsChunk l;
       l.base = c.base; l.npix = c.npix; l.s0 = c.s0; l.width = J.width;
-      l.seed = J.seed; l.max_depth = J.max_depth; l.scene = J.scene; l.pad = 0;
+      l.seed = J.seed; l.max_depth = J.max_depth; l.scene = J.scene; l.flags = J.flags;
       l.primary = J.primary; l.pad2 = nullptr;
       sch[i] = l;
    }
    for (uint32_t i = threadIdx.x; i < nscenes; i += blockDim.x) ssc[i] = scenes[i];
    __syncthreads();
    const uint32_t lane = __lane_id();
+   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
    const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
-                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, max_stack};
+                        gtid, gridDim.x * blockDim.x, max_stack};
    RaySource srcq{nslots, blockIdx.x % kSegments, 0u, 0u, 0u, 0u, nslots == 0};
    if (!srcq.done) srcq.request(work, lane);
    Trav s;
@@ -661,11 +739,28 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
    s.sp = 0;
    uint32_t mode = kPIdle;       // lane word
    uint32_t slot = 0;
-   uint32_t t0 = 0;              // Shade1 -> Shade2: mesh; Shade2 -> Shade3: brdf
-   float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2 -> Shade3: reflectance
+   uint32_t t0 = 0;              // Shade1 -> Shade2: mesh; Shade2 -> Shade3: brdf; Light/Shadow: next light
+   float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2 -> Shade3: reflectance; Light/Shadow: visible, last vertex
    float src[3] = {1.f, 1.f, 1.f};
+   float direct[3] = {0.f, 0.f, 0.f};    // next-event sum of the path (kNee)
+   float pend[3] = {0.f, 0.f, 0.f};      // the light term a running shadow ray decides (kNee)
    const Node64 *nodes = nullptr;
    const float *tris = nullptr;
+   // At a Lambert vertex of a next-event job: keep the vertex in this lane's
+   // record and go through the lights before tracing the bounce ray.
+   auto nee_vertex = [&](const Frame &fr, const float refl[3], const float wout[3], uint32_t next_mode) {
+      const float k0 = src[0] * refl[0], k1 = src[1] * refl[1], k2 = src[2] * refl[2];
+      (void)k0;
+      NeeRecord r;
+      r.a = make_float4(fr.lx, fr.ly, fr.lz, wout[0]);
+      r.b = make_float4(wout[1], wout[2], k0, k1);
+      r.c = make_float4(k2, fr.nx, fr.ny, fr.nz);
+      float4 *dst = nee_scratch + 3 * (size_t)gtid;
+      dst[0] = r.a; dst[1] = r.b; dst[2] = r.c;
+      t0 = 0u;
+      t1 = 0.f;
+      mode = lw_with_state(next_mode, kPLight);
+   };
    while (true) {
       uint64_t idle = __ballot((mode & kPModeMask) == kPIdle);
       if (idle && !srcq.done) {
@@ -687,17 +782,18 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
       const uint32_t m0 = mode & kPModeMask;
       if (!__any(m0 != kPIdle)) break;
       const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3;
-      const bool run_heavy =
-            (uint32_t)__popcll(__ballot(heavy)) >= shade_min || !__any(m0 == kPTrace || m0 == kPShade1);
-      // leaf steps gather the same way; lanes at interior nodes or in Shade1
-      // always step, so some lane progresses whenever any lane holds
-      const bool at_leaf = m0 == kPTrace && (s.cur & kLeafBit) != 0;
+      const bool tracing = m0 == kPTrace || m0 == kPShadow;
+      const bool run_heavy = (uint32_t)__popcll(__ballot(heavy)) >= shade_min ||
+                             !__any(tracing || m0 == kPShade1 || m0 == kPLight);
+      // leaf steps gather the same way; lanes at interior nodes, in Shade1 or in
+      // Light always step, so some lane progresses whenever any lane holds
+      const bool at_leaf = tracing && (s.cur & kLeafBit) != 0;
       const bool run_leaf = (uint32_t)__popcll(__ballot(at_leaf)) >= leaf_min ||
-                            !__any((m0 == kPTrace && !at_leaf) || m0 == kPShade1);
+                            !__any((tracing && !at_leaf) || m0 == kPShade1 || m0 == kPLight);
       const uint32_t m = (heavy && !run_heavy) || (at_leaf && !run_leaf) ? kPHold : m0;
       // ------------------------------------------------------------ fetch
       Fetched f;
-      if (m == kPTrace) {
+      if (m == kPTrace || m == kPShadow) {
          fetch(s, nodes, tris, f);
       } else if (m == kPPrim) {
          const uint32_t cf = lds_find_chunk(sch, nchunks, slot);
@@ -725,24 +821,80 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          const PTX_GLOBAL float *nrm = as_global(ssc[sch[lw_chunk(mode)].scene].normals + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) { f.q[k] = v[k]; f.q[9 + k] = nrm[k]; }
+      } else if (kNee && m == kPLight) {
+         const PTX_GLOBAL f32x4u *rec = as_global(reinterpret_cast<const f32x4u *>(nee_scratch + 3 * (size_t)gtid));
+         const f32x4u a = rec[0], b = rec[1], c = rec[2];
+         f.q[0] = a.x; f.q[1] = a.y; f.q[2] = a.z; f.q[3] = a.w;
+         f.q[4] = b.x; f.q[5] = b.y; f.q[6] = b.z; f.q[7] = b.w;
+         f.q[8] = c.x; f.q[9] = c.y; f.q[10] = c.z; f.q[11] = c.w;
+         const DevScene &S = ssc[sch[lw_chunk(mode)].scene];
+         if (t0 < S.num_lights) {
+            const PTX_GLOBAL float *lp = as_global(S.lights + 6 * (size_t)t0);
+            const f32x4u la = *reinterpret_cast<const PTX_GLOBAL f32x4u *>(lp);
+            const f32x2u lb = *reinterpret_cast<const PTX_GLOBAL f32x2u *>(lp + 4);
+            f.q[12] = la.x; f.q[13] = la.y; f.q[14] = la.z; f.q[15] = la.w;
+            f.q[16] = lb.x; f.q[17] = lb.y;
+         }
       }
       // ------------------------------------------------------------ compute
       if (m == kPTrace) {
          if (!advance<kLds>(s, f, st, (mode & kPDbg) != 0)) {
             if (s.hit == kNone) {
-               write_res(res, slot, miss_radiance(ssc[sch[lw_chunk(mode)].scene], src));
+               write_res(res, slot, fin<kNee>(miss_radiance(ssc[sch[lw_chunk(mode)].scene], src), direct, mode));
                mode = kPIdle;
             } else {
                mode = lw_with_state(mode, kPShade1);
             }
+         }
+      } else if (kNee && m == kPShadow) {
+         const bool more = advance<kLds>(s, f, st, false);
+         if (s.hit != kNone) {                    // occluded
+            t1 = 0.f;
+            mode = lw_with_state(mode, kPLight);
+         } else if (!more) {                      // nothing before the light
+            t1 = 1.f;
+            mode = lw_with_state(mode, kPLight);
+         }
+      } else if (kNee && m == kPLight) {
+         const DevScene &S = ssc[sch[lw_chunk(mode)].scene];
+         if (t1 != 0.f) {
+            direct[0] = direct[0] + pend[0];
+            direct[1] = direct[1] + pend[1];
+            direct[2] = direct[2] + pend[2];
+         }
+         t1 = 0.f;
+         if (t0 < S.num_lights) {
+            float l[3], dist, g;
+            const uint32_t j = t0;
+            t0 = j + 1;
+            if (light_setup(&f.q[12], f.q[0], f.q[1], f.q[2], f.q[9], f.q[10], f.q[11], l, dist, g)) {
+               pend[0] = f.q[6] * (f.q[15] * g);
+               pend[1] = f.q[7] * (f.q[16] * g);
+               pend[2] = f.q[8] * (f.q[17] * g);
+               make_ray(s.r, f.q[0], f.q[1], f.q[2], l[0], l[1], l[2]);
+               begin(s, S.root_box, S.root_ref, S.root_count, dist);
+               if (s.cur == kNone) t1 = 1.f;       // root box missed: nothing in the way
+               else mode = lw_with_state(mode, kPShadow);
+            }
+         } else if (t2 != 0.f) {                  // the path ends at this vertex
+            write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
+            mode = kPIdle;
+         } else {                                 // trace the bounce ray
+            make_ray(s.r, f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5]);
+            const uint32_t next = p_root<kNee>(S, s, res, slot, src, direct, mode);
+            mode = next == kPIdle ? kPIdle : lw_with_state(mode, next);
          }
       } else if (m == kPPrim) {
          const LdsChunk &c = sch[lw_chunk(mode)];
          const uint32_t local = slot - c.base;
          const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
          const uint32_t w0 = __float_as_uint(f.q[0]);
+         if (kNee) {
+            direct[0] = direct[1] = direct[2] = 0.f;
+            if (c.flags & PTX_JOB_NEE) mode |= kPNee;
+         }
          if (w0 & 1u) {
-            write_res(res, slot, Radiance{f.q[1], f.q[2], f.q[3]});
+            write_res(res, slot, fin<kNee>(Radiance{f.q[1], f.q[2], f.q[3]}, direct, mode));
             mode = kPIdle;
          } else {
             const DevScene &S = ssc[c.scene];
@@ -759,18 +911,27 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             fr.t2x = f.q[10]; fr.t2y = f.q[11]; fr.t2z = f.q[12];
             const float lin[3] = {f.q[13], f.q[14], f.q[15]};
             src[0] = src[1] = src[2] = 1.f;
+            const uint32_t x = pixel % c.width, y = pixel / c.width;
+            const uint32_t dbg = (x == 128 && y == 128) ? kPDbg : 0u;
+            const bool nee = kNee && (mode & kPNee) && M.brdf != PTX_BRDF_MIRROR && S.num_lights > 0;
+            const float kd[3] = {M.refl[0], M.refl[1], M.refl[2]};
+            const float s_before[3] = {src[0], src[1], src[2]};
             float wout[3];
             bounce(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
-            if (src_is_zero(src)) {
-               write_res(res, slot, Radiance{0.f, 0.f, 0.f});
+            nodes = S.nodes;
+            tris = S.tris;
+            if (nee) {
+               const float keep[3] = {src[0], src[1], src[2]};
+               src[0] = s_before[0]; src[1] = s_before[1]; src[2] = s_before[2];
+               nee_vertex(fr, kd, wout, (mode & ~kPModeMask) | dbg | (1u << kPBounceShift));
+               src[0] = keep[0]; src[1] = keep[1]; src[2] = keep[2];
+               t2 = src_is_zero(src) ? 1.f : 0.f;
+            } else if (src_is_zero(src)) {
+               write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
                mode = kPIdle;
             } else {
-               const uint32_t x = pixel % c.width, y = pixel / c.width;
-               const uint32_t dbg = (x == 128 && y == 128) ? kPDbg : 0u;
                make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
-               nodes = S.nodes;
-               tris = S.tris;
-               const uint32_t next = p_root(S, s, res, slot, src);
+               const uint32_t next = p_root<kNee>(S, s, res, slot, src, direct, mode);
                mode = next == kPIdle ? kPIdle : (mode & ~kPModeMask) | dbg | (1u << kPBounceShift) | next;
             }
          }
@@ -786,12 +947,12 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const int32_t is_light = __float_as_int(f.q[4]);
             const uint32_t has_normals = __float_as_uint(f.q[8]);
             if (is_light) {
-               write_res(res, slot, Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
-                                             (f.q[7] + 0.f) * src[2]});
+               write_res(res, slot, fin<kNee>(Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
+                                                       (f.q[7] + 0.f) * src[2]}, direct, mode));
                mode = kPIdle;
                go = false;
             } else if (b >= c.max_depth) {
-               write_res(res, slot, Radiance{0.f, 0.f, 0.f});
+               write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
                mode = kPIdle;
                go = false;
             } else if (has_normals && ssc[c.scene].normals) {
@@ -815,16 +976,25 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
             const uint32_t local = slot - c.base;
             const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
+            const DevScene &S = ssc[c.scene];
+            const bool nee = kNee && (mode & kPNee) && M.brdf != PTX_BRDF_MIRROR && S.num_lights > 0;
+            const float s_before[3] = {src[0], src[1], src[2]};
             float lin[3], wout[3];
             world_to_local(fr, d, lin);
             bounce(M, fr, lin, pixel, sample, b, c.seed, wout, src);
-            if (src_is_zero(src)) {
-               write_res(res, slot, Radiance{0.f, 0.f, 0.f});
+            if (nee) {
+               const float keep[3] = {src[0], src[1], src[2]};
+               src[0] = s_before[0]; src[1] = s_before[1]; src[2] = s_before[2];
+               nee_vertex(fr, M.refl, wout, mode + (1u << kPBounceShift));
+               src[0] = keep[0]; src[1] = keep[1]; src[2] = keep[2];
+               t2 = src_is_zero(src) ? 1.f : 0.f;
+            } else if (src_is_zero(src)) {
+               write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
                mode = kPIdle;
             } else {
                make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
-               const uint32_t next = p_root(ssc[c.scene], s, res, slot, src);
-               mode = next == kPIdle ? kPIdle : ((mode & ~kPModeMask) + (1u << kPBounceShift)) | next;
+               const uint32_t next = p_root<kNee>(S, s, res, slot, src, direct, mode);
+               mode = next == kPIdle ? kPIdle : lw_with_state(mode + (1u << kPBounceShift), next);
             }
          }
       }
@@ -845,7 +1015,8 @@ int render_blocks_per_cu(int which) {
    hipError_t e = hipSuccess;
    if (which == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_extend<kRenderLds>, 256, 0);
    else if (which == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_primary<kRenderLds>, 256, 0);
-   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds>, 256, 0);
+   else if (which == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, false>, 256, 0);
+   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, true>, 256, 0);
    return e == hipSuccess && b > 0 ? b : 1;
 }
 
@@ -892,9 +1063,11 @@ void launch_zero_floats(float *a, size_t n, hipStream_t st) {
 }
 
 int paths_blocks_per_cu() {
-   int b = 0;
-   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_paths<kRenderLds>, 256, 0);
-   return e == hipSuccess && b > 0 ? b : 1;
+   int a = 0, b = 0;
+   const hipError_t ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_paths<kRenderLds, false>, 256, 0);
+   const hipError_t eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_paths<kRenderLds, true>, 256, 0);
+   if (ea != hipSuccess || eb != hipSuccess) return 1;
+   return std::max(1, std::min(a, b));   // one grid and one scratch area serve both variants
 }
 
 uint32_t paths_max_chunks() { return kMaxRoundChunks; }
@@ -902,14 +1075,21 @@ uint32_t paths_max_scenes() { return dev::kMaxRoundScenes; }
 
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-                  uint32_t leaf_min, uint32_t grid, hipStream_t st) {
+                  uint32_t leaf_min, float4 *nee_scratch, uint32_t grid, hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
-   hipLaunchKernelGGL(dev::k_paths<kRenderLds>, dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, scenes,
-                      nscenes, res, work, spill, max_stack, shade_min, leaf_min);
+   if (nee_scratch)
+      hipLaunchKernelGGL((dev::k_paths<kRenderLds, true>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs,
+                         scenes, nscenes, res, work, spill, max_stack, shade_min, leaf_min, nee_scratch);
+   else
+      hipLaunchKernelGGL((dev::k_paths<kRenderLds, false>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs,
+                         scenes, nscenes, res, work, spill, max_stack, shade_min, leaf_min, nee_scratch);
 }
 
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {
-   hipLaunchKernelGGL(dev::k_mega<kRenderLds>, dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+   if (J.flags & PTX_JOB_NEE)
+      hipLaunchKernelGGL((dev::k_mega<kRenderLds, true>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+   else
+      hipLaunchKernelGGL((dev::k_mega<kRenderLds, false>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
 }
 
 }  // namespace ptx

@@ -56,6 +56,7 @@ JobDev job_dev(const ptx_job_desc &j, uint32_t scene_index) {
    J.max_depth = j.max_depth;
    J.seed = j.seed;
    J.scene = scene_index;
+   J.flags = j.flags;
    return J;
 }
 
@@ -66,6 +67,7 @@ int validate_job(const ptx_job_desc *j) {
       return set_error(PTX_ERR_INVALID, "job film exceeds 2^28 pixels");
    if (j->spp == 0) return set_error(PTX_ERR_INVALID, "job has zero samples per pixel");
    if (j->max_depth > 1000) return set_error(PTX_ERR_INVALID, "job max_depth above 1000");
+   if (j->flags & ~(uint32_t)PTX_JOB_NEE) return set_error(PTX_ERR_INVALID, "unknown job flags");
    return PTX_OK;
 }
 
@@ -80,7 +82,7 @@ void RenderEngine::release() {
    if (g_ab) (void)hipGraphExecDestroy(g_ab);
    if (g_ba) (void)hipGraphExecDestroy(g_ba);
    g_ab = g_ba = nullptr;
-   dfree(qa); dfree(qb); dfree(hits); dfree(res); dfree(words); dfree(spill);
+   dfree(qa); dfree(qb); dfree(hits); dfree(res); dfree(words); dfree(spill); dfree(nee_scratch);
    dfree(d_jobs); dfree(d_scenes); dfree(d_chunks);
    cap = qcap = 0;
    spill_entries = 0;
@@ -102,9 +104,10 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       ext_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(0));
       shade_grid = (uint32_t)(dev->num_cus * 8);
       mega_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(2));
+      mega_nee_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(3));
       paths_grid = (uint32_t)(dev->num_cus * paths_blocks_per_cu());
    }
-   const uint32_t stride_threads = std::max({ext_grid, mega_grid, paths_grid}) * 256;
+   const uint32_t stride_threads = std::max({ext_grid, mega_grid, mega_nee_grid, paths_grid}) * 256;
    const size_t need_spill = max_stack > lds ? (size_t)(max_stack - lds) * stride_threads : 0;
    int rc;
    if (paths > cap) {
@@ -237,13 +240,21 @@ int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
    return PTX_OK;
 }
 
-int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes) {
+int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes,
+                       bool nee) {
    uint32_t nslots = 0;
    for (const Chunk &c : r) nslots = std::max(nslots, c.base + c.k * c.npix);
+   if (nee && !regen_round((uint32_t)r.size(), nscenes))
+      return set_error(PTX_ERR_INVALID, "next-event estimation renders in the path-regeneration form only");
+   if (nee && !nee_scratch) {
+      const int rc = dalloc(&nee_scratch, (size_t)paths_grid * 256 * 3);
+      if (rc) return rc;
+   }
    if (regen_round((uint32_t)r.size(), nscenes)) {
       const uint32_t shade_min = dev->tuning.shade_lanes ? dev->tuning.shade_lanes : kDefaultShadeLanes;
       launch_paths(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, nscenes, res, words + kWork, spill, stack_cap,
-                   shade_min, leaf_lanes_of(dev, true), std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
+                   shade_min, leaf_lanes_of(dev, true), nee ? nee_scratch : nullptr,
+                   std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
       launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
       return PTX_OK;
    }
@@ -317,8 +328,12 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
    size_t off = 0;
    for (const auto &r : rounds) {
       uint32_t depth = 0;
-      for (const Chunk &c : r) depth = std::max(depth, jobs[c.job].desc.max_depth);
-      if ((rc = round(d_chunks + off, r, depth, (uint32_t)scenes.size()))) {
+      bool nee = false;
+      for (const Chunk &c : r) {
+         depth = std::max(depth, jobs[c.job].desc.max_depth);
+         nee |= (jobs[c.job].desc.flags & PTX_JOB_NEE) != 0;
+      }
+      if ((rc = round(d_chunks + off, r, depth, (uint32_t)scenes.size(), nee))) {
          (void)hipStreamSynchronize(st);
          cleanup();
          return rc;
@@ -348,7 +363,8 @@ int RenderEngine::run_plain(const ptx_scene *scene, const ptx_job_desc &desc, fl
    JobDev J = job_dev(desc, 0);
    J.film = film;
    launch_zero_floats(film, (size_t)npix * 3, st);
-   const uint32_t grid = std::min<uint32_t>(mega_grid, (npix + 255) / 256);
+   const uint32_t grid =
+         std::min<uint32_t>((desc.flags & PTX_JOB_NEE) ? mega_nee_grid : mega_grid, (npix + 255) / 256);
    for (uint32_t s = 0; s < desc.spp; s++) launch_mega(scene->d, J, s, spill, grid, st);
    launch_finalize(film, npix, desc.spp, st);
    hipError_t e = hipMemcpyAsync(film_host, film, (size_t)npix * 3 * sizeof(float), hipMemcpyDeviceToHost, st);

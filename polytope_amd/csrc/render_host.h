@@ -38,6 +38,7 @@ struct RenderEngine {
    float *res = nullptr;                    // 3 floats per path slot
    uint32_t *words = nullptr;
    uint4 *spill = nullptr;
+   float4 *nee_scratch = nullptr;           // per resident lane of the regeneration kernel, 48 bytes
    size_t spill_entries = 0;
    uint32_t stack_cap = 0;
    JobDev *d_jobs = nullptr;
@@ -46,7 +47,7 @@ struct RenderEngine {
    uint32_t scenes_cap = 0;
    Chunk *d_chunks = nullptr;
    uint32_t chunks_cap = 0;
-   uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0, paths_grid = 0;
+   uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0, mega_nee_grid = 0, paths_grid = 0;
    hipGraphExec_t g_ab = nullptr, g_ba = nullptr;
    bool graphs_stale = true;
    uint32_t graph_leaf = 0;                 // leaf_min the graphs were captured with
@@ -68,7 +69,8 @@ struct RenderEngine {
    int start_job(JobDev &J, const DevScene &S, uint32_t npix);
    // One round over the chunks in d_round (a device copy of r); the job and
    // scene tables must hold the round's jobs, the scene table nscenes entries.
-   int round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes);
+   // nee: some job of the round uses next-event estimation (regeneration only).
+   int round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes, bool nee);
    int finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host);
    void free_job(JobDev &J);
    uint32_t capacity_for(uint64_t total_paths, uint32_t max_npix, bool regen) const;

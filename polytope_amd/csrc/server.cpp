@@ -182,6 +182,7 @@ void ptx_server::loop() {
          chunks.clear();
          uint64_t used = 0;
          uint32_t depth = 0;
+         bool nee = false;
          const uint64_t share = paths / in_round.size();
          for (auto &j : in_round) {
             const uint32_t left = j->desc.spp - j->next_sample;
@@ -192,11 +193,12 @@ void ptx_server::loop() {
             chunks.push_back(Chunk{j->slot, j->next_sample, (uint32_t)k, j->npix, (uint32_t)used});
             used += k * j->npix;
             depth = std::max(depth, j->desc.max_depth);
+            nee |= (j->desc.flags & PTX_JOB_NEE) != 0;
          }
          if (!chunks.empty()) {
             (void)hipMemcpyAsync(eng.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice,
                                  stream);
-            round_rc = eng.round(eng.d_chunks, chunks, depth, kMaxServerScenes);
+            round_rc = eng.round(eng.d_chunks, chunks, depth, kMaxServerScenes, nee);
          }
          const hipError_t e = hipStreamSynchronize(stream);
          std::lock_guard<std::mutex> lock(mu);

@@ -70,8 +70,11 @@ __device__ __forceinline__ bool pop_next(Trav &s, const Stack<kLds> &st) {
    return false;
 }
 
-__device__ __forceinline__ void begin(Trav &s, const float *root_box, uint32_t root_ref, uint32_t root_count) {
-   s.min_t = kInf;
+// Starts a traversal of s.r from the root.  tmax < inf bounds it (shadow rays:
+// boxes and faces beyond tmax are culled as a closer hit would cull them).
+__device__ __forceinline__ void begin(Trav &s, const float *root_box, uint32_t root_ref, uint32_t root_count,
+                                      float tmax = kInf) {
+   s.min_t = tmax;
    s.hit = 0xFFFFFFFFu;
    s.after = 0;
    s.sp = 0;

@@ -138,3 +138,52 @@ def test_render_batch_more_jobs_than_lds_table(dev, mode):
     for i in (0, 1, 33, 64, 69):
         (alone,) = render_batch([jobs[i]])
         assert np.array_equal(alone.view(np.uint32), mixed[i].view(np.uint32))
+
+
+def _nee_scene():
+    v, f = S.icosphere(3, radius=1.0, center=(0.0, 1.0, 0.0), noise=0.05, seed=3)
+    gv, gf = S.quad(6.0, 0.0)
+    lv, lf = S.quad(1.0, 4.5)
+    meshes = [S.MeshData(v, f, brdf=S.MIRROR, refl=(0.9, 0.8, 0.7)),
+              S.MeshData(gv, gf, refl=(0.5, 0.6, 0.7)),
+              S.MeshData(*S.icosphere(2, radius=0.6, center=(1.8, 0.6, -0.5), noise=0.0), refl=(0.7, 0.3, 0.3)),
+              S.MeshData(lv, lf, is_light=True, emission=(255.0, 240.0, 200.0))]
+    lights = [((0.0, 3.0, -2.5), (3000.0, 2800.0, 2500.0)), ((-2.5, 1.5, 0.5), (600.0, 900.0, 1200.0)),
+              ((0.0, -1.0, 0.0), (5000.0, 5000.0, 5000.0))]      # the last one is under the floor
+    cam = S.Camera(S.look_at((0.0, 2.0, -5.0), (0.0, 0.8, 0.0)), 60.0, 144, 96)
+    return meshes, lights, cam, (40.0, 60.0, 90.0)
+
+
+@pytest.mark.parametrize("depth", [0, 1, 5])
+def test_next_event_estimation_matches_oracle(dev, depth):
+    meshes, lights, cam, sky = _nee_scene()
+    print(f"case nee depth={depth}", flush=True)
+    ref = O.OracleScene(meshes, sky, lights=lights).render(cam, 3, depth, seed=21, nee=True)
+    sc = Scene(dev, meshes, sky, lights=lights)
+    plain = render(sc, cam, 3, depth, 21, nee=True)
+    assert np.array_equal(plain.view(np.uint32), ref.view(np.uint32))
+    (batched,) = batch(dev, [(sc, cam, 3, depth, 21, True)], _abi.RENDER_REGEN)
+    assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
+    if depth:
+        off = O.OracleScene(meshes, sky, lights=lights).render(cam, 3, depth, seed=21)
+        assert (ref >= off).all() and (ref > off).any()
+
+
+def test_next_event_jobs_mixed_with_plain_jobs(dev):
+    # one round holding a next-event job and an ordinary job of the same scene:
+    # the ordinary job's film is the one it has without next-event jobs around
+    meshes, lights, cam, sky = _nee_scene()
+    sc = Scene(dev, meshes, sky, lights=lights)
+    small = S.Camera(cam.camera_to_world, 60.0, 64, 40)
+    mixed = render_batch([(sc, cam, 2, 4, 5, True), (sc, small, 3, 4, 6)])
+    alone = render_batch([(sc, small, 3, 4, 6)])[0]
+    assert np.array_equal(mixed[1].view(np.uint32), alone.view(np.uint32))
+    ref = O.OracleScene(meshes, sky, lights=lights).render(cam, 2, 4, seed=5, nee=True)
+    assert np.array_equal(mixed[0].view(np.uint32), ref.view(np.uint32))
+    # the wavefront form has no shadow stage: such jobs are refused, not rendered wrong
+    dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT)
+    try:
+        with pytest.raises(Exception):
+            render_batch([(sc, cam, 1, 2, 5, True)])
+    finally:
+        dev.set_tuning()

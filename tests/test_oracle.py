@@ -326,3 +326,46 @@ def test_stress_feature_rays_do_not_leak():
     sc = O.OracleScene(meshes)
     again = O.OracleScene(meshes, bvh=sc.bvh()).intersect(rays[:4096])
     assert np.array_equal(again.view(np.uint8), ref[:4096].view(np.uint8))
+
+
+def test_next_event_estimation_closed_form():
+    # one Lambert plane y = 0 (Kd), one point light straight above at height h, a
+    # camera looking down, depth 1 and no sky: a pixel's radiance is exactly the
+    # direct term Kd * I * cos / (pi * d^2) at its hit point (the bounce ray then
+    # escapes to black); a blocker between plane and light makes its shadow black
+    kd = (0.5, 0.6, 0.7)
+    intensity = (1000.0, 2000.0, 3000.0)
+    h = 3.0
+    gv, gf = S.quad(10.0, 0.0)
+    plane = S.MeshData(gv, gf, refl=kd)
+    cam = S.Camera(S.look_at((0.0, 6.0, 0.0), (0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0)), 60.0, 24, 24)
+    lights = [((0.0, h, 0.0), intensity)]
+    film = O.OracleScene([plane], None, lights=lights).render(cam, 1, 1, 3, nee=True)
+    # hit points of the pixel-centre rays on y = 0
+    rays = S.rays_coherent(cam, cam.width, cam.height).astype(np.float64)
+    t = -rays[:, 1] / rays[:, 4]
+    p = rays[:, :3] + t[:, None] * rays[:, 3:]
+    w = np.array([0.0, h, 0.0]) - p
+    d2 = (w * w).sum(1)
+    cosv = w[:, 1] / np.sqrt(d2)
+    expect = np.array(kd)[None, :] * np.array(intensity)[None, :] * (cosv / (np.pi * d2))[:, None]
+    got = film.reshape(-1, 3).astype(np.float64)
+    assert np.abs(got - expect).max() / expect.max() < 1e-4
+    # depth 0 stops before any vertex, so there is no direct term
+    assert not O.OracleScene([plane], None, lights=lights).render(cam, 1, 0, 3, nee=True).any()
+    # a small blocker at y = 1.5 under the light: plane points whose segment to the
+    # light crosses it (and that the camera still sees) are black, the others keep
+    # their value; points within 2 % of the shadow's edge are not checked
+    bv, bf = S.quad(0.5, 1.5)
+    film2 = O.OracleScene([plane, S.MeshData(bv, bf, refl=(0.1, 0.1, 0.1))], None,
+                          lights=lights).render(cam, 1, 1, 3, nee=True).reshape(-1, 3)
+    cam_o = rays[:, :3]
+    seen = np.abs(cam_o + (p - cam_o) * ((6.0 - 1.5) / 6.0))[:, [0, 2]].max(1) > 0.52    # camera ray misses the blocker
+    at = p + (np.array([0.0, h, 0.0]) - p) * (1.5 / h)                                 # light segment at y = 1.5
+    r = np.abs(at[:, [0, 2]]).max(1)
+    shadow, lit = seen & (r < 0.49), seen & (r > 0.51)
+    assert shadow.sum() >= 4 and lit.sum() >= 100
+    assert (film2[shadow] == 0).all()
+    assert np.array_equal(film2[lit], film.reshape(-1, 3)[lit])
+    # off by default: the same scene without the flag renders black
+    assert not O.OracleScene([plane], None, lights=lights).render(cam, 1, 1, 3).any()
