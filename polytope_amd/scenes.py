@@ -362,6 +362,125 @@ def write_ptxm(path: str, meshes: Sequence[MeshData], camera: Optional[Camera] =
         fh.write(bytes(out))
 
 
+def look_at_lh(eye, target, up) -> np.ndarray:
+    """Camera-to-world matrix exactly as the reference's pbrt scene block makes
+    it from ``LookAt``: Transform::LookAt(eye, lookAt, up, false)
+    (src/cpu/structures/Transform.cpp:322-365) composed onto the identity with
+    Matrix4x4::operator*= (src/cpu/structures/Matrix4x4.cpp:244-283), all in
+    float32 with the same operation order (so signed zeros match too)."""
+    f = np.float32
+    ex, ey, ez = (f(v) for v in eye)
+    tx, ty, tz = (f(v) for v in target)
+    ux, uy, uz = (f(v) for v in up)
+
+    def normalize(x, y, z):                                   # Vector::Normalize
+        ld = f(1.0) / f(np.sqrt(x * x + y * y + z * z))
+        return x * ld, y * ld, z * ld
+
+    def cross(a, b):                                          # Vector::Cross
+        return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+
+    with np.errstate(all="ignore"):
+        d = normalize(tx - ex, ty - ey, tz - ez)
+        u = normalize(ux, uy, uz)
+        left = cross(u, d)
+        if f(np.sqrt(left[0] * left[0] + left[1] * left[1] + left[2] * left[2])) == 0:
+            raise ValueError("LookAt: up is parallel to the view direction")
+        left = normalize(*left)
+        new_up = cross(d, left)
+        m = np.zeros((4, 4), np.float32)
+        m[0, 0], m[1, 0], m[2, 0] = left
+        m[0, 1], m[1, 1], m[2, 1] = new_up
+        m[0, 2], m[1, 2], m[2, 2] = d
+        m[0, 3], m[1, 3], m[2, 3], m[3, 3] = ex, ey, ez, f(1)
+        ident = np.eye(4, dtype=np.float32)
+        out = np.zeros((4, 4), np.float32)
+        for i in range(4):
+            for j in range(4):
+                acc = ident[i, 0] * m[0, j]
+                for k in range(1, 4):
+                    acc = f(acc + ident[i, k] * m[k, j])
+                out[i, j] = acc
+    return out
+
+
+def _fmt(x) -> str:
+    return "%.9g" % float(np.float32(x))
+
+
+def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov: float, width: int, height: int,
+               spp: int, max_depth: int, skybox: Optional[Sequence[float]] = None, name: str = "scene"):
+    """Writes ``name``.pbrt and one ASCII PLY per mesh into ``directory``, in the
+    subset of the format the reference's parser reads (pbrt_parser.cpp,
+    mesh_parsers.cpp): LookAt, Camera "perspective", Film, Sampler, Integrator
+    "path", LightSource "infinite", per mesh AttributeBegin / Translate / Scale /
+    Material "matte" | "mirror" / AreaLightSource "diffuse" / Shape "plymesh".
+    Meshes must have a translate-and-scale object-to-world (no rotation), which
+    the reference composes exactly.  Floats are written with 9 significant
+    digits (exact for float32).  Emission and skybox are written as value / 255
+    because the reference multiplies by 255 (pbrt_parser.cpp:1281-1283, 1340).
+
+    Returns (path, meshes, camera, skybox) as the reference will see them:
+    emission and sky after the / 255 * 255 round trip and the camera built by
+    look_at_lh, so that rendering them here and there gives the same film."""
+    import os
+    f = np.float32
+    os.makedirs(directory, exist_ok=True)
+    lines = ["LookAt " + " ".join(_fmt(v) for v in (*eye, *target, *up)),
+             'Camera "perspective" "float fov" [ %s ]' % _fmt(fov),
+             'Film "image" "integer xresolution" [ %d ] "integer yresolution" [ %d ] "string filename" [ "%s.png" ]'
+             % (width, height, name),
+             'Sampler "halton" "integer pixelsamples" [ %d ]' % spp,
+             'Integrator "path" "integer maxdepth" [ %d ]' % max_depth,
+             "WorldBegin"]
+    sky_out = None
+    if skybox is not None:
+        l = [f(f(v) / f(255)) for v in skybox]
+        lines.append('LightSource "infinite" "rgb L" [ %s ]' % " ".join(_fmt(v) for v in l))
+        sky_out = tuple(float(f(v) * f(255)) for v in l)
+    out_meshes = []
+    for i, m in enumerate(meshes):
+        o2w = np.asarray(m.object_to_world, np.float32)
+        lin = o2w[:3, :3]
+        if np.any(lin != np.diag(np.diag(lin))) or np.any(o2w[3] != np.array([0, 0, 0, 1], np.float32)):
+            raise ValueError(f"mesh {i}: write_pbrt supports translate-and-scale transforms only")
+        ply = f"{name}_mesh{i}.ply"
+        v = np.asarray(m.verts, np.float32)
+        fc = np.asarray(m.faces, np.uint32)
+        with open(os.path.join(directory, ply), "w") as fh:
+            fh.write("ply\nformat ascii 1.0\nelement vertex %d\nproperty float x\nproperty float y\n"
+                     "property float z\nelement face %d\nproperty list uchar int vertex_indices\nend_header\n"
+                     % (len(v), len(fc)))
+            fh.write("".join("%s %s %s\n" % (_fmt(a), _fmt(b), _fmt(c)) for a, b, c in v))
+            fh.write("".join("3 %d %d %d\n" % (a, b, c) for a, b, c in fc))
+        lines.append("AttributeBegin")
+        t = o2w[:3, 3]
+        s = np.diag(lin)
+        if np.any(t != 0):
+            lines.append("  Translate " + " ".join(_fmt(x) for x in t))
+        if np.any(s != 1):
+            lines.append("  Scale " + " ".join(_fmt(x) for x in s))
+        if m.brdf == MIRROR:
+            lines.append('  Material "mirror" "rgb Kr" [ %s ]' % " ".join(_fmt(x) for x in m.refl))
+        else:
+            lines.append('  Material "matte" "rgb Kd" [ %s ]' % " ".join(_fmt(x) for x in m.refl))
+        emission = m.emission
+        if m.is_light:
+            l = [f(f(x) / f(255)) for x in m.emission]
+            lines.append('  AreaLightSource "diffuse" "rgb L" [ %s ]' % " ".join(_fmt(x) for x in l))
+            emission = tuple(float(f(x) * f(255)) for x in l)
+        lines.append('  Shape "plymesh" "string filename" [ "%s" ]' % ply)
+        lines.append("AttributeEnd")
+        out_meshes.append(MeshData(v, fc, object_to_world=o2w, brdf=m.brdf,
+                                   refl=tuple(float(f(x)) for x in m.refl), is_light=m.is_light, emission=emission))
+    lines.append("WorldEnd")
+    path = os.path.join(directory, name + ".pbrt")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    cam = Camera(look_at_lh(eye, target, up), float(f(fov)), width, height)
+    return path, out_meshes, cam, sky_out
+
+
 def write_rays(path: str, rays: np.ndarray) -> None:
     rays = np.ascontiguousarray(rays, np.float32)
     with open(path, "wb") as fh:
