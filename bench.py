@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--render-paths", type=int, default=0, help="path slots per round (0 = default)")
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-rate", type=float, default=8.0)
+    ap.add_argument("--serving-loaded-requests", type=int, default=64,
+                    help="second stream at a rate near the GPU's capacity")
+    ap.add_argument("--serving-loaded-rate", type=float, default=64.0)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
@@ -205,11 +208,11 @@ def plain_cpu_render_section(dev, args, threads: int):
     return out
 
 
-def serving_section(dev, args):
+def serving_section(dev, args, n_requests: int, rate: float):
     """Poisson stream of render jobs through the continuous-batching server."""
     from polytope_amd import Scene, Server
     from polytope_amd import scenes as S
-    scene_defs, reqs = S.serving_requests(args.serving_requests, args.serving_rate)
+    scene_defs, reqs = S.serving_requests(n_requests, rate)
     scenes_ = [Scene(dev, m, sky) for m, sky in scene_defs]
     srv = Server(dev, hbm_budget_bytes=64 << 30)
     ids = []
@@ -230,7 +233,8 @@ def serving_section(dev, args):
     for s_ in scenes_:
         s_.close()
     lat = np.array(lat)
-    return {"requests": len(reqs), "rate_hz": args.serving_rate,
+    offered = sum(cam.width * cam.height * spp for _, _, cam, spp, _, _ in reqs) / max(1e-9, reqs[-1][0])
+    return {"requests": len(reqs), "rate_hz": rate, "offered_samples_per_s": round(offered, 1),
             "scenes_tris": [sum(m.num_faces for m in d[0]) for d in scene_defs],
             "throughput_samples_per_s": round(samples / (last - first), 1),
             "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
@@ -348,7 +352,8 @@ def main():
         out["render"] = render_section(dev, args)
         out["render_cpu_runnable"] = plain_cpu_render_section(dev, args, args.cpu_threads or min(16, os.cpu_count() or 1))
     if not args.skip_serving:
-        out["serving"] = serving_section(dev, args)
+        out["serving"] = serving_section(dev, args, args.serving_requests, args.serving_rate)
+        out["serving_loaded"] = serving_section(dev, args, args.serving_loaded_requests, args.serving_loaded_rate)
     print(json.dumps(out), flush=True)
     if args.calibrate_copy_mb:
         src = torch.empty(args.calibrate_copy_mb << 20, dtype=torch.uint8, device="cuda")

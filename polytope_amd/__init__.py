@@ -236,8 +236,9 @@ def render_batch(jobs: Sequence[tuple]) -> List[np.ndarray]:
 
 
 def autotune(device: Device, scene: Scene) -> dict:
-    """Times traversal (LDS stack depth x refill threshold) and wavefront path
-    capacity on ``scene`` and keeps the fastest settings on ``device``."""
+    """Times candidate tunings on ``scene`` and keeps the fastest on ``device``:
+    traversal LDS stack depth x refill threshold (ray queries), then round form x
+    path slots per round, shading and leaf gathering thresholds (rendering)."""
     t = _abi.Tuning()
     check(load().ptx_autotune(device.handle, scene.handle, C.byref(t)))
     return {k: getattr(t, k) for k, _ in _abi.Tuning._fields_}

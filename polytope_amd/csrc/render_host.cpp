@@ -570,6 +570,23 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
             }
          }
          best.shade_lanes = best_shade;
+         // 4. lanes a traversal step gathers at leaves (render kernels)
+         dev->tuning.shade_lanes = best_shade;
+         uint32_t best_leaf = 0;
+         for (uint32_t leaf : {1u, 8u, 16u, 32u}) {
+            dev->tuning.leaf_lanes = leaf;
+            double t_min = 1e30;
+            for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
+               const double t0 = now_s();
+               rc = engine_of(dev)->run_batch(rj, {scene}, films);
+               t_min = std::min(t_min, now_s() - t0);
+            }
+            if (rc == PTX_OK && t_min < best_s) {
+               best_s = t_min;
+               best_leaf = leaf;
+            }
+         }
+         best.leaf_lanes = best_leaf;
       }
    }
    (void)hipEventDestroy(e0);

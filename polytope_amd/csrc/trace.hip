@@ -48,6 +48,7 @@ __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, 
 template <bool kExtra>
 struct RayQuery {
    struct Ctx {};
+   static constexpr bool kStaged = true;
    DevScene sc;
    const float *rays;
    ptx_hit *hits;
@@ -67,6 +68,7 @@ struct RayQuery {
    __device__ __forceinline__ void begin_root(Ctx &, Trav &s) const {
       begin(s, sc.root_box, sc.root_ref, sc.root_count);
    }
+   __device__ __forceinline__ uint32_t root_count() const { return sc.root_count; }
    __device__ __forceinline__ const Node64 *nodes(const Ctx &) const { return sc.nodes; }
    __device__ __forceinline__ const float *tris(const Ctx &) const { return sc.tris; }
    __device__ __forceinline__ bool debug_pixel(const Ctx &) const { return false; }
@@ -76,7 +78,7 @@ struct RayQuery {
 };
 
 template <int kLds, bool kExtra>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n, ptx_hit *__restrict__ hits,
               ptx_ray_counters *counters, ptx_hit_frame *frames, uint32_t *work, uint4 *spill, uint32_t refill_min,
               uint32_t leaf_min) {

@@ -62,8 +62,8 @@ __device__ __forceinline__ float next_down(float x) {
 struct Ray {
    float ox, oy, oz, dx, dy, dz;
    float ix, iy, iz;          // 1 / d  (bvh.cpp:477-481)
-   uint32_t neg;              // bit a: inverse_direction[a] < 0 (bvh.cpp:483)
-   uint32_t kz;               // Wald dominant axis; kx = kz+1, ky = kz+2 (mod 3)
+   uint32_t nk;               // bit a (0-2): inverse_direction[a] < 0 (bvh.cpp:483); bits 3-4: the
+                              // Wald dominant axis kz, with kx = kz+1, ky = kz+2 (mod 3)
    float Sx, Sy, Sz;
 };
 
@@ -74,12 +74,12 @@ __device__ __forceinline__ float pick3(uint32_t k, float a, float b, float c) {
 __device__ __forceinline__ void make_ray(Ray &r, float ox, float oy, float oz, float dx, float dy, float dz) {
    r.ox = ox; r.oy = oy; r.oz = oz; r.dx = dx; r.dy = dy; r.dz = dz;
    r.ix = 1.f / dx; r.iy = 1.f / dy; r.iz = 1.f / dz;
-   r.neg = (r.ix < 0 ? 1u : 0u) | (r.iy < 0 ? 2u : 0u) | (r.iz < 0 ? 4u : 0u);
+   const uint32_t neg = (r.ix < 0 ? 1u : 0u) | (r.iy < 0 ? 2u : 0u) | (r.iz < 0 ? 4u : 0u);
    const float ax = __builtin_fabsf(dx), ay = __builtin_fabsf(dy), az = __builtin_fabsf(dz);
    uint32_t kz = 0;                                            // mesh.cpp:200-204
    if (ay > ax && ay > az) kz = 1;
    else if (az > ax && az > ay) kz = 2;
-   r.kz = kz;
+   r.nk = neg | kz << 3;
    const uint32_t kx = kz == 2 ? 0 : kz + 1;
    const uint32_t ky = kx == 2 ? 0 : kx + 1;
    const float pdx = pick3(kx, dx, dy, dz), pdy = pick3(ky, dx, dy, dz), pdz = pick3(kz, dx, dy, dz);
@@ -130,7 +130,7 @@ __device__ __forceinline__ bool test_triangle(const float *v, const Ray &r, floa
    const float a0 = v0x - r.ox, a1 = v0y - r.oy, a2 = v0z - r.oz;
    const float b0 = v1x - r.ox, b1 = v1y - r.oy, b2 = v1z - r.oz;
    const float c0 = v2x - r.ox, c1 = v2y - r.oy, c2 = v2z - r.oz;
-   const uint32_t kz = r.kz;
+   const uint32_t kz = r.nk >> 3;
    // kz = 0 -> (y, z, x); kz = 1 -> (z, x, y); kz = 2 -> (x, y, z)
    float p0x = pick3(kz, a1, a2, a0), p0y = pick3(kz, a2, a0, a1), p0z = pick3(kz, a0, a1, a2);
    float p1x = pick3(kz, b1, b2, b0), p1y = pick3(kz, b2, b0, b1), p1z = pick3(kz, b0, b1, b2);

@@ -9,6 +9,8 @@
 // per lane), deeper entries spill to a global array sized from the tree depth.
 #pragma once
 
+#include <type_traits>
+
 #include "trace_device.h"
 
 #pragma clang fp contract(off)
@@ -177,7 +179,7 @@ __device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<k
    const bool h0 = in0 || (n0 <= f0 && n0 <= s.min_t);
    const bool h1 = in1 || (n1 <= f1 && n1 <= s.min_t);
    // bvh.cpp:510-517: with neg_dir[axis] the high child (child 1) is popped first.
-   const bool high_first = (s.r.neg >> maxis) & 1u;
+   const bool high_first = (s.r.nk >> maxis) & 1u;
    const bool fh = high_first ? h1 : h0, sh = high_first ? h0 : h1;
    const uint32_t c0 = mcounts & 0xFFFFu, c1 = mcounts >> 16;
    const uint32_t fref = high_first ? mref1 : mref0, sref = high_first ? mref0 : mref1;
@@ -242,6 +244,38 @@ struct RaySource {
 
 enum : uint32_t { kIdle = 0, kPending = 1, kStarting = 2, kActive = 3, kHeld = 4 };
 
+// Reads v from lane `from` of the wave (every lane active).
+__device__ __forceinline__ uint32_t lane_read(uint32_t from, uint32_t v) {
+   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(from << 2), (int)v);
+}
+__device__ __forceinline__ float lane_read(uint32_t from, float v) {
+   return __uint_as_float(lane_read(from, __float_as_uint(v)));
+}
+
+// Policies whose rays start from their id alone (no per-lane context, no
+// second load) set kStaged: their ray set-up is then done wave-wide.
+template <class P, class = void>
+struct staged_policy {
+   static constexpr bool value = false;
+};
+template <class P>
+struct staged_policy<P, decltype(void(P::kStaged))> {
+   static constexpr bool value = P::kStaged;
+};
+
+// Ray set-up (reciprocal direction, Wald shear, root box test: six correctly
+// rounded divisions and a slab test) done for a whole batch of ids at once:
+// lane j of the wave loads and prepares ray base + j, and lanes that need a
+// ray later copy a prepared one from its lane.  Set up per refilling lane it
+// would run for the whole wave at almost every step, since some lane starts a
+// ray at almost every step.
+struct Staged {
+   float ox, oy, oz, dx, dy, dz, ix, iy, iz, Sx, Sy, Sz;
+   uint32_t nk;
+   uint32_t cur;                      // root reference, or kNone when the root box is missed
+   uint32_t base, left, off;          // wave-uniform: batch base id, rays not yet handed out, next one
+};
+
 // Runs the persistent loop.  Policy supplies:
 //   Ctx                                   per-lane work context
 //   load(id, f)                           issue the loads of work item id into f.q
@@ -272,38 +306,112 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
    typename Policy::Ctx ctx;
    uint32_t mode = kIdle;
    uint32_t id = 0;
+   refill_min = min(refill_min, 64u);     // a wave that is all idle always refills
+   constexpr bool kStaged = staged_policy<Policy>::value;
+   static_assert(!kStaged || std::is_empty<typename Policy::Ctx>::value, "staged policies carry no context");
+   Staged g;
+   g.ox = g.oy = g.oz = g.dx = g.dy = g.dz = g.ix = g.iy = g.iz = g.Sx = g.Sy = g.Sz = 0.f;
+   g.nk = 0;
+   g.cur = 0xFFFFFFFFu;
+   g.left = 0;
+   g.base = g.off = 0;
    while (true) {
       uint64_t idle = __ballot(mode == kIdle);
-      if (idle && !src.done && (uint32_t)__popcll(idle) >= refill_min) {
-         while (idle) {
-            src.refill(work, lane);
-            if (src.done) break;
-            const uint32_t take = min((uint32_t)__popcll(idle), src.pool_left);
-            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
-            const bool mine = ((idle >> lane) & 1ull) && rank < take;
-            if (mine) {
-               id = src.pool_base + rank;
-               mode = kPending;
+      if constexpr (kStaged) {
+         if (idle && !(src.done && g.left == 0) && (uint32_t)__popcll(idle) >= refill_min) {
+            while (idle) {
+               if (g.left == 0) {
+                  src.refill(work, lane);
+                  if (src.done) break;
+                  g.base = src.pool_base;
+                  g.left = src.pool_left;
+                  g.off = 0;
+                  src.pool_left = 0;
+                  if (lane < g.left) {
+                     Fetched f;
+                     Trav t;
+                     typename Policy::Ctx c;
+                     P.load(g.base + lane, f);
+                     P.start(g.base + lane, f, c, t);
+                     P.begin_root(c, t);
+                     g.ox = t.r.ox; g.oy = t.r.oy; g.oz = t.r.oz; g.dx = t.r.dx; g.dy = t.r.dy; g.dz = t.r.dz;
+                     g.ix = t.r.ix; g.iy = t.r.iy; g.iz = t.r.iz; g.Sx = t.r.Sx; g.Sy = t.r.Sy; g.Sz = t.r.Sz;
+                     g.nk = t.r.nk;
+                     g.cur = t.cur;
+                  }
+               }
+               const uint32_t take = min((uint32_t)__popcll(idle), g.left);
+               const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+               const bool mine = ((idle >> lane) & 1ull) && rank < take;
+               const uint32_t from = g.off + min(rank, take - 1u);
+               Ray r;
+               r.ox = lane_read(from, g.ox); r.oy = lane_read(from, g.oy); r.oz = lane_read(from, g.oz);
+               r.dx = lane_read(from, g.dx); r.dy = lane_read(from, g.dy); r.dz = lane_read(from, g.dz);
+               r.ix = lane_read(from, g.ix); r.iy = lane_read(from, g.iy); r.iz = lane_read(from, g.iz);
+               r.Sx = lane_read(from, g.Sx); r.Sy = lane_read(from, g.Sy); r.Sz = lane_read(from, g.Sz);
+               const uint32_t nk = lane_read(from, g.nk), cur = lane_read(from, g.cur);
+               r.nk = nk;
+               if (mine) {
+                  // the state begin() leaves, with the root test already done
+                  id = g.base + from;
+                  s.r = r;
+                  s.min_t = kInf;
+                  s.hit = 0xFFFFFFFFu;
+                  s.after = 0;
+                  s.sp = 0;
+                  s.tris = 0;
+                  s.hu = s.hv = s.hw = 0.f;
+                  s.cur = cur;
+                  s.cnt = cur != 0xFFFFFFFFu ? P.root_count() : 0u;
+                  s.bb_hits = cur != 0xFFFFFFFFu ? 1u : 0u;
+                  s.bb_misses = 1u - s.bb_hits;
+                  mode = kActive;
+                  if (cur == 0xFFFFFFFFu) {
+                     P.finish(id, ctx, s);
+                     mode = kIdle;
+                  }
+               }
+               idle &= ~__ballot(mine);
+               g.off += take;
+               g.left -= take;
             }
-            idle &= ~__ballot(mine);
-            src.pool_base += take;
-            src.pool_left -= take;
+         }
+      } else {
+         if (idle && !src.done && (uint32_t)__popcll(idle) >= refill_min) {
+            while (idle) {
+               src.refill(work, lane);
+               if (src.done) break;
+               const uint32_t take = min((uint32_t)__popcll(idle), src.pool_left);
+               const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+               const bool mine = ((idle >> lane) & 1ull) && rank < take;
+               if (mine) {
+                  id = src.pool_base + rank;
+                  mode = kPending;
+               }
+               idle &= ~__ballot(mine);
+               src.pool_base += take;
+               src.pool_left -= take;
+            }
          }
       }
-      if (!__any(mode != kIdle)) break;
+      if (!__any(mode != kIdle)) {
+         // staged lanes whose ray missed the root finished at once: more may be left
+         if (kStaged && !(src.done && g.left == 0)) continue;
+         break;
+      }
       const bool at_leaf = mode == kActive && (s.cur & kLeafBit) != 0;
       const bool hold = at_leaf && (uint32_t)__popcll(__ballot(at_leaf)) < leaf_min &&
                         __any(mode != kIdle && !at_leaf);
       const uint32_t m = hold ? kHeld : mode;
       Fetched f;
-      if (m == kPending) {
+      if (!kStaged && m == kPending) {
          P.load(id, f);
-      } else if (m == kStarting) {
+      } else if (!kStaged && m == kStarting) {
          P.load2(ctx, f);
       } else if (m == kActive) {
          fetch(s, P.nodes(ctx), P.tris(ctx), f);
       }
-      if (m == kPending) {
+      if (!kStaged && m == kPending) {
          const int r = P.start(id, f, ctx, s);
          if (r == 2) {
             P.finish(id, ctx, s);
@@ -318,7 +426,7 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
                mode = kIdle;
             }
          }
-      } else if (m == kStarting) {
+      } else if (!kStaged && m == kStarting) {
          P.start2(ctx, f, s);
          mode = kActive;
          if (s.cur == 0xFFFFFFFFu) {
