@@ -173,6 +173,12 @@ def reference_available() -> bool:
     return REF_DRIVER.exists()
 
 
+def reference_scene(rel: str) -> str:
+    """Path of one of the reference's own scene files (scenes/<rel>), read in
+    place by the scene-file tests where the reference tree is present."""
+    return str(REFERENCE_ROOT / "scenes" / rel)
+
+
 def run_reference(args: Sequence[str], cwd: Optional[str] = None, timeout: int = 600) -> subprocess.CompletedProcess:
     return subprocess.run([str(REF_DRIVER), *args], cwd=cwd, capture_output=True, text=True, timeout=timeout,
                           check=True)

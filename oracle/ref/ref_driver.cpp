@@ -63,9 +63,93 @@ struct HitRecord {
 };
 static_assert(sizeof(HitRecord) == 100, "hit record layout");
 
+// Everything the reference's parser produced, as JSON with each float as its
+// 32-bit pattern: film size, samples, integrator depth, sampler, camera,
+// skybox and, per Scene::Shapes entry in order, its transforms, light or
+// material and packed geometry (pbrt_parser.cpp:907-1721, mesh_parsers.cpp).
+void describe(const poly::AbstractRunner& runner, const std::string& path) {
+   std::FILE* f = std::fopen(path.c_str(), "w");
+   if (!f) { std::fprintf(stderr, "ref_driver: cannot write %s\n", path.c_str()); std::exit(1); }
+   auto bits = [](float v) { uint32_t u; std::memcpy(&u, &v, 4); return u; };
+   auto farr = [&](const float* v, size_t n) {
+      std::fputc('[', f);
+      for (size_t i = 0; i < n; i++) std::fprintf(f, "%s%u", i ? "," : "", bits(v[i]));
+      std::fputc(']', f);
+   };
+   auto uarr = [&](const std::vector<unsigned int>& v) {
+      std::fputc('[', f);
+      for (size_t i = 0; i < v.size(); i++) std::fprintf(f, "%s%u", i ? "," : "", v[i]);
+      std::fputc(']', f);
+   };
+   auto mat = [&](const poly::Transform& t) { farr(&t.Matrix.Matrix[0][0], 16); };
+   const auto* integrator = dynamic_cast<const poly::PathTraceIntegrator*>(runner.Integrator.get());
+   const auto* camera = dynamic_cast<const poly::PerspectiveCamera*>(runner.Scene->Camera.get());
+   std::fprintf(f, "{\"width\": %u, \"height\": %u, \"spp\": %u, \"max_depth\": %d, \"center_sampler\": %s,\n",
+                runner.Bounds.x, runner.Bounds.y, runner.NumSamples, integrator ? (int)integrator->MaxDepth : -1,
+                dynamic_cast<const poly::CenterSampler*>(runner.Sampler.get()) ? "true" : "false");
+   std::fprintf(f, " \"camera\": {\"c2w\": ");
+   mat(camera->CameraToWorld);
+   std::fprintf(f, ", \"fov\": %u},\n \"sky\": ", bits(camera->Settings.FieldOfView));
+   if (const auto* sky = dynamic_cast<const poly::ColorSkybox*>(runner.Scene->Skybox.get())) {
+      const poly::SpectralPowerDistribution s = sky->GetSpd(poly::Vector(0.f, 0.f, 1.f));
+      const float rgb[3] = {s.r, s.g, s.b};
+      farr(rgb, 3);
+   } else {
+      std::fprintf(f, "null");
+   }
+   std::fprintf(f, ",\n \"meshes\": [");
+   for (size_t i = 0; i < runner.Scene->Shapes.size(); i++) {
+      const poly::Mesh& m = *runner.Scene->Shapes[i];
+      const poly::mesh_geometry& g = *m.mesh_geometry;
+      std::fprintf(f, "%s\n  {\"o2w\": ", i ? "," : "");
+      mat(*m.object_to_world);
+      std::fprintf(f, ", \"w2o\": ");
+      mat(*m.world_to_object);
+      std::fprintf(f, ", \"light\": %s", m.spd ? "true" : "false");
+      if (m.spd) {
+         const float e[3] = {m.spd->r, m.spd->g, m.spd->b};
+         std::fprintf(f, ", \"spd\": ");
+         farr(e, 3);
+      }
+      const poly::AbstractBRDF* brdf = m.material ? m.material->BRDF.get() : nullptr;
+      const char* kind = "none";
+      const poly::ReflectanceSpectrum* refl = nullptr;
+      if (const auto* lam = dynamic_cast<const poly::LambertBRDF*>(brdf)) { kind = "lambert"; refl = &lam->refl; }
+      else if (const auto* mir = dynamic_cast<const poly::MirrorBRDF*>(brdf)) { kind = "mirror"; refl = &mir->refl; }
+      else if (brdf) kind = "other";
+      std::fprintf(f, ", \"brdf\": \"%s\"", kind);
+      if (refl) {
+         const float r[3] = {refl->r, refl->g, refl->b};
+         std::fprintf(f, ", \"refl\": ");
+         farr(r, 3);
+      }
+      std::fprintf(f, ", \"has_normals\": %s, \"x\": ", g.has_vertex_normals ? "true" : "false");
+      farr(g.x_packed.data(), g.x_packed.size());
+      std::fprintf(f, ", \"y\": ");
+      farr(g.y_packed.data(), g.y_packed.size());
+      std::fprintf(f, ", \"z\": ");
+      farr(g.z_packed.data(), g.z_packed.size());
+      std::fprintf(f, ", \"nx\": ");
+      farr(g.nx_packed.data(), g.nx_packed.size());
+      std::fprintf(f, ", \"ny\": ");
+      farr(g.ny_packed.data(), g.ny_packed.size());
+      std::fprintf(f, ", \"nz\": ");
+      farr(g.nz_packed.data(), g.nz_packed.size());
+      std::fprintf(f, ", \"fv0\": ");
+      uarr(g.fv0);
+      std::fprintf(f, ", \"fv1\": ");
+      uarr(g.fv1);
+      std::fprintf(f, ", \"fv2\": ");
+      uarr(g.fv2);
+      std::fputc('}', f);
+   }
+   std::fprintf(f, "\n ]}\n");
+   std::fclose(f);
+}
+
 void usage() {
    std::fprintf(stderr,
-      "usage: ref_driver (--ptxm scene.bin | --pbrt scene.pbrt) [--nodes out] [--pairs out]\n"
+      "usage: ref_driver (--ptxm scene.bin | --pbrt scene.pbrt) [--nodes out] [--pairs out] [--describe out.json]\n"
       "                  [--rays rays.bin --hits out] [--render out] [--threads N]\n"
       "                  [--time-rays rays.bin --threads N --repeat R] [--time-render --threads N]\n");
    std::exit(2);
@@ -74,7 +158,7 @@ void usage() {
 }  // namespace
 
 int main(int argc, char** argv) {
-   std::string ptxm, pbrt, nodes_out, pairs_out, rays_in, hits_out, render_out, time_rays;
+   std::string ptxm, pbrt, nodes_out, pairs_out, rays_in, hits_out, render_out, time_rays, describe_out;
    bool png = false;
    bool time_render = false;
    unsigned threads = 1, repeat = 1;
@@ -93,6 +177,7 @@ int main(int argc, char** argv) {
       else if (a == "--repeat") repeat = (unsigned)std::atoi(next().c_str());
       else if (a == "--time-render") time_render = true;
       else if (a == "--png") png = true;   // after --render: PNGFilm::Output into the working directory
+      else if (a == "--describe") describe_out = next();
       else usage();
    }
    Job job;
@@ -108,6 +193,12 @@ int main(int argc, char** argv) {
       usage();
    }
    poly::Scene* scene = runner->Scene;
+   if (!describe_out.empty()) {
+      describe(*runner, describe_out);
+      if (nodes_out.empty() && pairs_out.empty() && rays_in.empty() && render_out.empty() && time_rays.empty() &&
+          !time_render)
+         return 0;
+   }
 
    const auto t0 = std::chrono::steady_clock::now();
    scene->bvh_root.bound(scene->Shapes);

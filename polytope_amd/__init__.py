@@ -19,6 +19,7 @@ import numpy as np
 from . import _abi
 from ._abi import (BVH_NODE_DTYPE, COUNTER_DTYPE, FRAME_DTYPE, HIT_DTYPE, PtxError, check, load)
 from .scenes import Camera, MeshData
+from .pbrt import PbrtError, SceneFile, read_pbrt, read_pbrt_string
 
 __all__ = ["Device", "Scene", "PtxError", "render", "render_batch", "Server", "autotune", "HIT_DTYPE", "COUNTER_DTYPE", "FRAME_DTYPE", "BVH_NODE_DTYPE",
            "mesh_descs", "build_bvh", "camera_desc", "job_desc", "MeshData", "Camera"]

@@ -35,12 +35,15 @@ class MeshData:
     refl: Tuple[float, float, float] = (0.5, 0.5, 0.5)
     is_light: bool = False
     emission: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    w2o: Optional[np.ndarray] = None       # world-to-object if given (the inverse of object_to_world otherwise)
 
     @property
     def num_faces(self) -> int:
         return int(self.faces.shape[0])
 
     def world_to_object(self) -> np.ndarray:
+        if self.w2o is not None:
+            return np.asarray(self.w2o, np.float32)
         return np.linalg.inv(self.object_to_world.astype(np.float64)).astype(np.float32)
 
 
