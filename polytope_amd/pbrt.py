@@ -729,6 +729,11 @@ class SceneFile:
     # a textured or glossy one); they carry the default Lambert 0.5 material
     unrenderable: List[Tuple[int, str]] = field(default_factory=list)
 
+    @property
+    def lights(self) -> List[Tuple[Tuple[float, float, float], Tuple[float, float, float]]]:
+        """The point lights in the form Scene(..., lights=) takes."""
+        return [(l.position, l.intensity) for l in self.point_lights]
+
 
 def _first_int(d: Directive) -> Optional[Arg]:
     for a in d.args:

@@ -43,3 +43,18 @@ def test_gpu_renders_scene_file_like_oracle(dev, name, tmp_path):
     assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
     assert (ref != ref[0, 0]).any(2).mean() > 0.01         # the scene is visible
     sc.close()
+
+
+def test_gpu_renders_scene_file_point_light_like_oracle(dev, tmp_path):
+    print("case pbrt scene point light, next-event estimation", flush=True)
+    _crafted_meshes(tmp_path)
+    path = tmp_path / "s.pbrt"
+    path.write_text(CRAFTED["point_light"])
+    sf = P.read_pbrt(str(path))
+    sc = Scene(dev, sf.meshes, sf.skybox, lights=sf.lights)
+    film = render(sc, sf.camera, 4, sf.max_depth, 3, nee=True)
+    ref = O.OracleScene(sf.meshes, sf.skybox, lights=sf.lights).render(sf.camera, 4, sf.max_depth, 3, nee=True)
+    plain = O.OracleScene(sf.meshes, sf.skybox).render(sf.camera, 4, sf.max_depth, 3)
+    assert np.array_equal(film.view(np.uint32), ref.view(np.uint32))
+    assert (ref > plain).any()                             # the light adds to the film
+    sc.close()

@@ -152,6 +152,22 @@ Rotate -15 0 1 0
 ObjectInstance "o"
 WorldEnd
 """,
+    "point_light": """
+LookAt 0 2 -6  0 0 0  0 1 0
+Camera "perspective" "float fov" [ 40 ]
+Film "image" "integer xresolution" [ 48 ] "integer yresolution" [ 32 ]
+Integrator "path" "integer maxdepth" [ 3 ]
+WorldBegin
+LightSource "infinite" "rgb L" [ .05 .05 .05 ]
+AttributeBegin
+  Translate 0 0.5 0
+  LightSource "point" "rgb I" [ 8 7 6 ] "point from" [ 0 3 -1 ]
+AttributeEnd
+Material "matte" "rgb Kd" [ .7 .7 .7 ]
+Shape "plymesh" "string filename" [ "ico.ply" ]
+Shape "objmesh" "string filename" [ "quad.obj" ]
+WorldEnd
+""",
     "defaults_and_stacks": """
 LookAt 0 0 -5 0 0 0 0 1 0
 WorldBegin
@@ -253,6 +269,16 @@ def test_reference_scene_files(rel, tmp_path):
 def test_reference_scene_files_the_reference_rejects(rel, what):
     with pytest.raises(P.PbrtError, match=what):
         P.read_pbrt(O.reference_scene(rel))
+
+
+def test_point_lights_are_read_for_next_event_estimation(tmp_path):
+    _crafted_meshes(tmp_path)
+    path = tmp_path / "s.pbrt"
+    path.write_text(CRAFTED["point_light"])
+    sf = P.read_pbrt(str(path))
+    i255 = tuple(float(np.float32(np.float32(x) * np.float32(255))) for x in (8, 7, 6))
+    assert sf.lights == [((0.0, 3.5, -1.0), i255)]
+    assert len(sf.meshes) == 2                     # the point light makes no mesh
 
 
 def test_unrenderable_meshes_are_refused_when_strict(tmp_path):
