@@ -45,7 +45,9 @@ enum {
    PTX_ERR_NO_DEVICE = 7     /* no usable gfx950 device                       */
 };
 
-enum { PTX_BRDF_LAMBERT = 1, PTX_BRDF_MIRROR = 2 };
+/* PTX_BRDF_GLOSSY: GlossyBRDF, the reference's "plastic" material
+ * (src/cpu/shading/brdf/glossy_brdf.h:13-60, pbrt_parser.cpp:791-842). */
+enum { PTX_BRDF_LAMBERT = 1, PTX_BRDF_MIRROR = 2, PTX_BRDF_GLOSSY = 3 };
 
 typedef struct ptx_device ptx_device;
 typedef struct ptx_scene ptx_scene;
@@ -61,10 +63,12 @@ typedef struct ptx_mesh_desc {
    const uint32_t *fv0, *fv1, *fv2;      /* per-face vertex indices             */
    float object_to_world[16];
    float world_to_object[16];            /* Transform::Inverse, used for normals */
-   int32_t brdf;                         /* PTX_BRDF_LAMBERT or PTX_BRDF_MIRROR  */
-   float refl[3];                        /* Kd (Lambert) or Kr (mirror)          */
+   int32_t brdf;                         /* PTX_BRDF_LAMBERT, _MIRROR or _GLOSSY */
+   float refl[3];                        /* Kd (Lambert, glossy) or Kr (mirror)  */
    int32_t is_light;                     /* Mesh::spd != nullptr                 */
    float emission[3];                    /* *Mesh::spd, 0-255 units              */
+   float specular[3];                    /* glossy: Ks                           */
+   float roughness;                      /* glossy: 0 = mirror-like .. 1        */
 } ptx_mesh_desc;
 
 /* poly::compact_bvh_node, byte-compatible (src/cpu/acceleration/bvh.h:24-53). */

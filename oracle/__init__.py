@@ -74,6 +74,11 @@ def lib() -> C.CDLL:
         L.orc_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32]
         L.orc_philox4x32_10.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_acos.restype = C.c_float
+        L.orc_acos.argtypes = [C.c_float]
+        L.orc_glossy_sample.restype = C.c_int
+        L.orc_glossy_sample.argtypes = [C.c_void_p, C.c_float, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
+                                        C.POINTER(C.c_uint32)]
         L.orc_box_hits.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.POINTER(C.c_int)]
         L.orc_camera_ray.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
         _lib = L
@@ -159,6 +164,24 @@ def sincos(theta: float) -> Tuple[float, float]:
     s, c = C.c_float(), C.c_float()
     lib().orc_sincos(C.c_float(theta), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def acos(x: float) -> float:
+    """The project-owned acos the glossy lobe uses (CPU and GPU alike)."""
+    return lib().orc_acos(C.c_float(x))
+
+
+def glossy_sample(incoming, roughness: float, uniforms, libm: bool):
+    """GlossyBRDF::sample on given uniforms (u0, u1 per attempt); libm selects
+    the C library's acos/sin/cos over the project-owned ones.  Returns
+    (direction or None, attempts drawn)."""
+    v = np.ascontiguousarray(incoming, np.float32)
+    u = np.ascontiguousarray(uniforms, np.float32)
+    out = np.zeros(3, np.float32)
+    used = C.c_uint32()
+    ok = lib().orc_glossy_sample(v.ctypes.data, C.c_float(roughness), u.ctypes.data, len(u) // 2, 1 if libm else 0,
+                                 out.ctypes.data, C.byref(used))
+    return (out if ok else None), used.value
 
 
 def box_hits(box, ray, min_t: float) -> bool:

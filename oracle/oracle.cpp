@@ -19,6 +19,7 @@
 //   local frame          Intersection::WorldToLocal/LocalToWorld src/cpu/structures/Intersection.cpp:10-23
 //   BRDFs                LambertBRDF/AbstractBRDF/MirrorBRDF::sample
 //                        src/cpu/shading/brdf/{lambert_brdf.h:48-57,abstract_brdf.h:36-45,mirror_brdf.h:23-35}
+//                        GlossyBRDF::sample (the "plastic" material)  src/cpu/shading/brdf/glossy_brdf.h:27-60
 //   box filter           BoxFilter::AddSample/Output            src/cpu/filters/BoxFilter.cpp:14-71
 //   runner               TileRunner::Run 2x2 tiles              src/cpu/runners/TileRunner.cpp:61-117
 //
@@ -185,7 +186,11 @@ struct MeshIn {            // mirrors polytope_amd's ptx_mesh_desc (C layout, se
    float refl[3];
    int32_t is_light;
    float emission[3];
+   float specular[3];                // glossy: Ks (refl holds Kd)
+   float roughness;                  // glossy: 0 specular .. 1 diffuse
 };
+
+static_assert(sizeof(MeshIn) == 256, "MeshIn mirrors ptx_mesh_desc");
 
 struct Mesh {
    uint32_t nf = 0;
@@ -193,7 +198,8 @@ struct Mesh {
    bool has_normals = false;
    std::vector<V3> vn;                 // world-transformed vertex normals per face corner
    int brdf = 1;
-   float refl[3] = {1, 1, 1};
+   float refl[3] = {1, 1, 1};        // glossy: the mixed reflectance Ks (1 - r) + Kd r
+   float rough = 0;                  // glossy roughness
    bool light = false;
    float L[3] = {0, 0, 0};
 };
@@ -665,7 +671,105 @@ static inline void ptx_sincos(float theta, float* s_out, float* c_out) {
    }
 }
 
+// acos for the glossy lobe, project-owned so that CPU and GPU agree bitwise:
+// acos(x) = pi/2 - asin(x) for |x| <= 1/2, else 2 asin(sqrt((1 - |x|)/2)), taken
+// from pi for x < 0; asin(t) = t + t^3 P(t^2) with P the Maclaurin series of
+// (asin t - t) / t^3, coefficients (2n)! / (4^n (n!)^2 (2n + 1)) for n = 1..12,
+// in Horner form with every step an explicit fma.  |x| > 1 gives NaN, as acosf.
+static inline float asin_series(float t, float z) {
+   float p = std::fmaf(z, 0.006447210311889649f, 0.0073125258735988454f);
+   p = std::fmaf(z, p, 0.008390335809616815f);
+   p = std::fmaf(z, p, 0.009761609529194078f);
+   p = std::fmaf(z, p, 0.011551800896139705f);
+   p = std::fmaf(z, p, 0.01396484375f);
+   p = std::fmaf(z, p, 0.017352764423076924f);
+   p = std::fmaf(z, p, 0.022372159090909092f);
+   p = std::fmaf(z, p, 0.030381944444444444f);
+   p = std::fmaf(z, p, 0.044642857142857144f);
+   p = std::fmaf(z, p, 0.075f);
+   p = std::fmaf(z, p, 0.16666666666666666f);
+   return std::fmaf(t * z, p, t);
+}
+
+static inline float ptx_acos(float x) {
+   const float a = std::fabs(x);
+   if (a <= 0.5f) return 1.57079637050628662109f - asin_series(x, x * x);
+   const float z = (1.0f - a) * 0.5f;
+   const float r = 2.0f * asin_series(std::sqrt(z), z);
+   return x < 0.0f ? 3.14159274101257324219f - r : r;
+}
+
+// GlossyBRDF::sample (src/cpu/shading/brdf/glossy_brdf.h:27-60): a cosine-
+// hemisphere sample rotated about (mirror direction x normal) by -acos(mirror . n),
+// then mixed with the mirror direction by the roughness; repeated with new
+// uniforms while the result points below the surface.  The reference loops
+// without bound; here attempt a draws uniforms draw(a) and the lobe gives up
+// (returns false) after kGlossyTries attempts.  The result is not normalised, as
+// in the reference (LocalToWorld normalises it).  Trig supplies acos and
+// sin/cos: the project-owned ones for rendering, libm for pinning against the
+// reference's own function.
+constexpr uint32_t kGlossyTries = 64;
+
+struct PtxTrig {
+   static float acos_(float x) { return ptx_acos(x); }
+   static void sincos_(float t, float* s, float* c) { ptx_sincos(t, s, c); }
+};
+struct LibmTrig {
+   static float acos_(float x) { return std::acos(x); }
+   static void sincos_(float t, float* s, float* c) { *s = std::sin(t); *c = std::cos(t); }
+};
+
+template <class Trig, class Draw>
+static bool glossy_sample(V3 in, float rough, Draw draw, V3& out, uint32_t* used = nullptr) {
+   for (uint32_t a = 0; a < kGlossyTries; a++) {
+      if (used) *used = a + 1;
+      float u0, u1;
+      draw(a, u0, u1);
+      const float r = std::sqrt(u0);                          // CosineSampleHemisphere
+      const float theta = kTwoPI * u1;
+      float sn, cs;
+      Trig::sincos_(theta, &sn, &cs);
+      const V3 lam = {r * cs, std::sqrt(std::max(0.0f, 1.0f - u0)), r * sn};
+      const float factor = (0.f * in.x + 1.f * in.y + 0.f * in.z) * 2;   // incoming . n, n = (0, 1, 0)
+      V3 spec = {in.x - 0.f * factor, in.y - 1.f * factor, in.z - 0.f * factor};
+      spec = normalize(spec);
+      const float angle = -Trig::acos_(0.f * spec.x + 1.f * spec.y + 0.f * spec.z);
+      const float x = spec.y * 0.f - spec.z * 1.f;            // spec x n (Vector::Cross(Normal))
+      const float y = spec.z * 0.f - spec.x * 0.f;
+      const float z = spec.x * 1.f - spec.y * 0.f;
+      float s, c;                                              // Transform::Rotate (Transform.cpp:214-228)
+      Trig::sincos_(angle, &s, &c);
+      const float omc = 1 - c;
+      const float m00 = c + x * x * omc, m01 = x * y * omc - z * s, m02 = x * z * omc + y * s;
+      const float m10 = y * x * omc + z * s, m11 = c + y * y * omc, m12 = y * z * omc - x * s;
+      const float m20 = z * x * omc - y * s, m21 = z * y * omc + x * s, m22 = c + z * z * omc;
+      const V3 rot = {lam.x * m00 + lam.y * m01 + lam.z * m02,   // Transform::Apply(Vector)
+                      lam.x * m10 + lam.y * m11 + lam.z * m12,
+                      lam.x * m20 + lam.y * m21 + lam.z * m22};
+      const float k = 1 - rough;
+      const V3 o = {spec.x * k + rot.x * rough, spec.y * k + rot.y * rough, spec.z * k + rot.z * rough};
+      if (o.y < 0) continue;
+      out = o;
+      return true;
+   }
+   return false;
+}
+
 // ----------------------------------------------------------------------------- render
+
+// A mesh's shading data.  Glossy meshes keep the reflectance GlossyBRDF::sample
+// returns, Ks (1 - r) + Kd r per channel (glossy_brdf.h:52), which is constant.
+static void set_material(Mesh& m, const MeshIn& in) {
+   m.brdf = in.brdf;
+   std::memcpy(m.refl, in.refl, sizeof(m.refl));
+   if (in.brdf == 3) {
+      const float r = in.roughness;
+      for (int c = 0; c < 3; c++) m.refl[c] = in.specular[c] * (1 - r) + in.refl[c] * r;
+      m.rough = r;
+   }
+   m.light = in.is_light != 0;
+   std::memcpy(m.L, in.emission, sizeof(m.L));
+}
 
 struct CameraIn {                 // mirrors ptx_camera_desc
    float camera_to_world[16];
@@ -750,7 +854,7 @@ static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t
       const Mesh& m = s.meshes[is.mesh];
       if (m.light) return done({(m.L[0] + 0.f) * sr, (m.L[1] + 0.f) * sg, (m.L[2] + 0.f) * sb});
       if (bounces >= max_depth) return done({0, 0, 0});
-      if (nee && m.brdf != 2) add_direct(s, is, m.refl, sr, sg, sb, direct);
+      if (nee && m.brdf == 1) add_direct(s, is, m.refl, sr, sg, sb, direct);
       const uint32_t bounce_index = bounces;
       bounces++;
       // Intersection::WorldToLocal (Intersection.cpp:10-14); bent normal == geo normal
@@ -761,6 +865,20 @@ static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t
       if (m.brdf == 2) {
          rr = m.refl[0]; rg = m.refl[1]; rb = m.refl[2];
          lout = {lin.x, -lin.y, lin.z};
+      } else if (m.brdf == 3) {
+         rr = m.refl[0]; rg = m.refl[1]; rb = m.refl[2];
+         const uint32_t key[2] = {seed, kRngKeyHi};
+         auto draw = [&](uint32_t a, float& u0, float& u1) {
+            const uint32_t ctr[4] = {pixel_index, sample_index, bounce_index, a};
+            uint32_t rnd[4];
+            philox4x32_10(ctr, key, rnd);
+            u0 = u01(rnd[0]);
+            u1 = u01(rnd[1]);
+         };
+         if (!glossy_sample<PtxTrig>(lin, m.rough, draw, lout)) {
+            sr = sg = sb = 0.f;                 // no accepted direction: the path ends black
+            continue;
+         }
       } else {
          rr = m.refl[0]; rg = m.refl[1]; rb = m.refl[2];
          const uint32_t ctr[4] = {pixel_index, sample_index, bounce_index, 0u};
@@ -816,10 +934,7 @@ orc::Scene* orc_scene_create(const orc::MeshIn* meshes, uint32_t n_meshes, int h
                m.vn[3 * f + c] = orc::apply_normal(in.world_to_object, {in.nx[ix[c]], in.ny[ix[c]], in.nz[ix[c]]});
          }
       }
-      m.brdf = in.brdf;
-      std::memcpy(m.refl, in.refl, sizeof(m.refl));
-      m.light = in.is_light != 0;
-      std::memcpy(m.L, in.emission, sizeof(m.L));
+      set_material(m, in);
    }
    s->has_sky = has_sky != 0;
    if (sky) std::memcpy(s->sky, sky, sizeof(s->sky));
@@ -851,10 +966,7 @@ orc::Scene* orc_scene_create_prebuilt(const orc::MeshIn* meshes, uint32_t n_mesh
                m.vn[3 * f + c] = orc::apply_normal(in.world_to_object, {in.nx[ix[c]], in.ny[ix[c]], in.nz[ix[c]]});
          }
       }
-      m.brdf = in.brdf;
-      std::memcpy(m.refl, in.refl, sizeof(m.refl));
-      m.light = in.is_light != 0;
-      std::memcpy(m.L, in.emission, sizeof(m.L));
+      set_material(m, in);
    }
    s->has_sky = has_sky != 0;
    if (sky) std::memcpy(s->sky, sky, sizeof(s->sky));
@@ -990,6 +1102,28 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
    orc::philox4x32_10(ctr, key, out);
 }
 void orc_sincos(float theta, float* s, float* c) { orc::ptx_sincos(theta, s, c); }
+
+float orc_acos(float x) { return orc::ptx_acos(x); }
+
+// GlossyBRDF::sample on given uniforms (pairs u0, u1 per attempt); use_libm
+// selects libm's acos/sin/cos (as the reference calls them) over the
+// project-owned ones.  Returns 1 and the direction when an attempt is accepted
+// within n_pairs attempts, else 0; *used = attempts drawn.
+int orc_glossy_sample(const float in[3], float roughness, const float* uniforms, uint32_t n_pairs, int use_libm,
+                      float out[3], uint32_t* used) {
+   const orc::V3 v = {in[0], in[1], in[2]};
+   auto draw = [&](uint32_t a, float& u0, float& u1) {
+      if (a < n_pairs) { u0 = uniforms[2 * a]; u1 = uniforms[2 * a + 1]; }
+      else { u0 = 0.f; u1 = 0.f; }
+   };
+   orc::V3 o = {0, 0, 0};
+   uint32_t u = 0;
+   const bool ok = use_libm ? orc::glossy_sample<orc::LibmTrig>(v, roughness, draw, o, &u)
+                            : orc::glossy_sample<orc::PtxTrig>(v, roughness, draw, o, &u);
+   out[0] = o.x; out[1] = o.y; out[2] = o.z;
+   *used = u;
+   return ok && u <= n_pairs ? 1 : 0;
+}
 void orc_box_hits(const float box[6], const float ray[6], float min_t, int* hit) {
    const orc::Box b = {{box[0], box[1], box[2]}, {box[3], box[4], box[5]}};
    const orc::V3 o = {ray[0], ray[1], ray[2]};

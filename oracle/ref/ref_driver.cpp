@@ -37,6 +37,10 @@
 #include "cpu/samplers/samplers.h"
 #include "cpu/scenes/Scene.h"
 #include "cpu/scenes/skyboxes/ColorSkybox.h"
+#include <cassert>                       // glossy_brdf.h uses assert without including it
+#include <cmath>
+#include <random>
+#include "cpu/shading/brdf/glossy_brdf.h"
 #include "cpu/shading/brdf/lambert_brdf.h"
 #include "cpu/shading/brdf/mirror_brdf.h"
 #include "cpu/shapes/mesh.h"
@@ -116,6 +120,7 @@ void describe(const poly::AbstractRunner& runner, const std::string& path) {
       const poly::ReflectanceSpectrum* refl = nullptr;
       if (const auto* lam = dynamic_cast<const poly::LambertBRDF*>(brdf)) { kind = "lambert"; refl = &lam->refl; }
       else if (const auto* mir = dynamic_cast<const poly::MirrorBRDF*>(brdf)) { kind = "mirror"; refl = &mir->refl; }
+      else if (dynamic_cast<const poly::GlossyBRDF*>(brdf)) kind = "glossy";     // its parameters are private
       else if (brdf) kind = "other";
       std::fprintf(f, ", \"brdf\": \"%s\"", kind);
       if (refl) {
@@ -147,6 +152,42 @@ void describe(const poly::AbstractRunner& runner, const std::string& path) {
    std::fclose(f);
 }
 
+// GlossyBRDF::sample (glossy_brdf.h:27-60) on recorded randomness: for each
+// case (incoming xyz, roughness, seed) this translation unit's generator
+// (constants.h:86-104, a per-translation-unit static) is seeded, the lobe is
+// sampled, and a copy of the generator is replayed through the same
+// distribution until it reaches the same state, which lists the uniforms the
+// sample consumed.  Output per case: outgoing xyz, count, up to 128 uniforms.
+void glossy_cases(const std::string& in_path, const std::string& out_path) {
+   Reader r(in_path.c_str());
+   const uint32_t n = r.u32();
+   std::vector<float> out;
+   for (uint32_t i = 0; i < n; i++) {
+      float v[4];
+      r.raw(v, sizeof(v));
+      const uint32_t seed = r.u32();
+      poly::Generator.seed(seed);
+      std::mt19937 replay = poly::Generator;
+      const poly::ReflectanceSpectrum one(1.f, 1.f, 1.f);
+      poly::GlossyBRDF brdf(one, one, v[3]);
+      poly::ReflectanceSpectrum refl;
+      float pdf = 0;
+      const poly::Vector o = brdf.sample(poly::Vector(v[0], v[1], v[2]), 0.f, 0.f, refl, pdf);
+      std::vector<float> used;
+      while (!(replay == poly::Generator) && used.size() < 4096) {
+         std::uniform_real_distribution<float> d(0.0f, 1.0f);
+         used.push_back(d(replay));
+      }
+      float rec[3 + 1 + 128] = {};
+      rec[0] = o.x; rec[1] = o.y; rec[2] = o.z;
+      const uint32_t cnt = (uint32_t)used.size();
+      std::memcpy(&rec[3], &cnt, 4);
+      for (size_t k = 0; k < used.size() && k < 128; k++) rec[4 + k] = used[k];
+      out.insert(out.end(), rec, rec + 132);
+   }
+   write_file(out_path, out.data(), out.size() * 4);
+}
+
 void usage() {
    std::fprintf(stderr,
       "usage: ref_driver (--ptxm scene.bin | --pbrt scene.pbrt) [--nodes out] [--pairs out] [--describe out.json]\n"
@@ -159,6 +200,7 @@ void usage() {
 
 int main(int argc, char** argv) {
    std::string ptxm, pbrt, nodes_out, pairs_out, rays_in, hits_out, render_out, time_rays, describe_out;
+   std::string glossy_in, glossy_out;
    bool png = false;
    bool time_render = false;
    unsigned threads = 1, repeat = 1;
@@ -178,7 +220,12 @@ int main(int argc, char** argv) {
       else if (a == "--time-render") time_render = true;
       else if (a == "--png") png = true;   // after --render: PNGFilm::Output into the working directory
       else if (a == "--describe") describe_out = next();
+      else if (a == "--glossy-cases") { glossy_in = next(); glossy_out = next(); }
       else usage();
+   }
+   if (!glossy_in.empty()) {
+      glossy_cases(glossy_in, glossy_out);
+      return 0;
    }
    Job job;
    std::unique_ptr<poly::AbstractRunner> runner;

@@ -57,6 +57,8 @@ def mesh_descs(meshes: Sequence[MeshData]) -> Tuple[C.Array, list]:
         d.refl[:] = [float(x) for x in m.refl]
         d.is_light = 1 if m.is_light else 0
         d.emission[:] = [float(x) for x in m.emission]
+        d.specular[:] = [float(x) for x in m.specular]
+        d.roughness = float(m.roughness)
     return arr, keep
 
 

@@ -35,7 +35,8 @@ class MeshDesc(C.Structure):
                 ("fv0", _up), ("fv1", _up), ("fv2", _up),
                 ("object_to_world", C.c_float * 16), ("world_to_object", C.c_float * 16),
                 ("brdf", C.c_int32), ("refl", C.c_float * 3),
-                ("is_light", C.c_int32), ("emission", C.c_float * 3)]
+                ("is_light", C.c_int32), ("emission", C.c_float * 3),
+                ("specular", C.c_float * 3), ("roughness", C.c_float)]
 
 
 class BvhNode(C.Structure):

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -43,6 +44,8 @@ int upload(ptx_scene *s, const std::vector<T> &v, const T **dst) {
 }
 
 }  // namespace
+
+static_assert(sizeof(ptx_mesh_desc) == 256, "ptx_mesh_desc layout (Python and oracle mirrors)");
 
 uint32_t lds_depth_of(const ptx_device *dev) { return dev->tuning.lds_stack ? dev->tuning.lds_stack : 12; }
 
@@ -141,8 +144,10 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
       const ptx_mesh_desc &m = desc->meshes[i];
       if (!m.x || !m.y || !m.z || !m.fv0 || !m.fv1 || !m.fv2)
          return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " lacks vertex or face arrays");
-      if (m.brdf != PTX_BRDF_LAMBERT && m.brdf != PTX_BRDF_MIRROR)
+      if (m.brdf != PTX_BRDF_LAMBERT && m.brdf != PTX_BRDF_MIRROR && m.brdf != PTX_BRDF_GLOSSY)
          return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " has an unknown brdf");
+      if (m.brdf == PTX_BRDF_GLOSSY && !std::isfinite(m.roughness))
+         return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " has a non-finite roughness");
       for (uint32_t f = 0; f < m.num_faces; f++)
          if (m.fv0[f] >= m.num_vertices || m.fv1[f] >= m.num_vertices || m.fv2[f] >= m.num_vertices)
             return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " face " + std::to_string(f) +
@@ -153,6 +158,8 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    auto s = std::make_unique<ptx_scene>();
    s->dev = dev;
    s->num_meshes = desc->num_meshes;
+   for (uint32_t i = 0; i < desc->num_meshes; i++)
+      s->has_glossy |= desc->meshes[i].brdf == PTX_BRDF_GLOSSY && !desc->meshes[i].is_light;
    std::vector<WorldMesh> meshes(desc->num_meshes);
    for (uint32_t i = 0; i < desc->num_meshes; i++) world_vertices(desc->meshes[i], meshes[i]);
    double build_s = 0;

@@ -35,7 +35,8 @@ static_assert(sizeof(Node64) == 64, "Node64 is 64 bytes");
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 
-// Per-mesh shading record.
+// Per-mesh shading record.  Glossy meshes that are not lights keep the lobe's
+// constant reflectance in refl and its roughness in emission[0].
 struct MeshRec {
    int32_t brdf;
    float refl[3];

@@ -82,6 +82,34 @@ __device__ __forceinline__ void sincos_poly(float theta, float &s_out, float &c_
    }
 }
 
+// acos for the glossy lobe, project-owned so that the GPU and the oracle agree
+// bitwise: acos(x) = pi/2 - asin(x) for |x| <= 1/2, else 2 asin(sqrt((1 - |x|)/2)),
+// taken from pi for x < 0; asin(t) = t + t^3 P(t^2), P the Maclaurin series of
+// (asin t - t) / t^3 (coefficients (2n)! / (4^n (n!)^2 (2n + 1)), n = 1..12) in
+// Horner form with explicit fma.  |x| > 1 gives NaN, as acosf does.
+__device__ __forceinline__ float asin_series(float t, float z) {
+   float p = fmaf_(z, 0.006447210311889649f, 0.0073125258735988454f);
+   p = fmaf_(z, p, 0.008390335809616815f);
+   p = fmaf_(z, p, 0.009761609529194078f);
+   p = fmaf_(z, p, 0.011551800896139705f);
+   p = fmaf_(z, p, 0.01396484375f);
+   p = fmaf_(z, p, 0.017352764423076924f);
+   p = fmaf_(z, p, 0.022372159090909092f);
+   p = fmaf_(z, p, 0.030381944444444444f);
+   p = fmaf_(z, p, 0.044642857142857144f);
+   p = fmaf_(z, p, 0.075f);
+   p = fmaf_(z, p, 0.16666666666666666f);
+   return fmaf_(t * z, p, t);
+}
+
+__device__ __forceinline__ float acos_poly(float x) {
+   const float a = __builtin_fabsf(x);
+   if (a <= 0.5f) return 1.57079637050628662109f - asin_series(x, x * x);
+   const float z = (1.0f - a) * 0.5f;
+   const float r = 2.0f * asin_series(__builtin_sqrtf(z), z);
+   return x < 0.0f ? 3.14159274101257324219f - r : r;
+}
+
 __device__ __forceinline__ void normalize3(float &x, float &y, float &z) {   // Vector::Normalize
    const float ld = 1.0f / __builtin_sqrtf(x * x + y * y + z * z);
    x *= ld; y *= ld; z *= ld;
@@ -125,14 +153,64 @@ __device__ __forceinline__ Radiance miss_radiance(const DevScene &S, const float
    return L;
 }
 
+// GlossyBRDF::sample (glossy_brdf.h:27-60): a cosine-hemisphere sample rotated
+// about (mirror direction x n) by -acos(mirror . n), n = (0, 1, 0), then mixed
+// with the mirror direction by the roughness; drawn again (uniforms of counter
+// word 3 = attempt) while it points below the surface.  The reference loops
+// without bound; after kGlossyTries attempts this returns false and the path
+// ends black.  The result is not normalised (LocalToWorld does that).
+constexpr uint32_t kGlossyTries = 64;
+
+__device__ __forceinline__ bool glossy_sample(const float in[3], float rough, uint32_t pixel, uint32_t sample,
+                                              uint32_t bounce_index, uint32_t seed, float &lx, float &ly,
+                                              float &lz) {
+   for (uint32_t a = 0; a < kGlossyTries; a++) {
+      uint32_t r0, r1;
+      philox4x32_10(pixel, sample, bounce_index, a, seed, kRngKeyHi, r0, r1);
+      const float u0 = u01(r0), u1 = u01(r1);
+      const float r = __builtin_sqrtf(u0);                   // CosineSampleHemisphere
+      const float theta = kTwoPI * u1;
+      float sn, cs;
+      sincos_poly(theta, sn, cs);
+      const float om = 1.0f - u0;
+      const float hx = r * cs, hy = __builtin_sqrtf(0.0f < om ? om : 0.0f), hz = r * sn;
+      const float factor = (0.f * in[0] + 1.f * in[1] + 0.f * in[2]) * 2;
+      float sx = in[0] - 0.f * factor, sy = in[1] - 1.f * factor, sz = in[2] - 0.f * factor;
+      normalize3(sx, sy, sz);
+      const float angle = -acos_poly(0.f * sx + 1.f * sy + 0.f * sz);
+      const float x = sy * 0.f - sz * 1.f, y = sz * 0.f - sx * 0.f, z = sx * 1.f - sy * 0.f;
+      float s, c;                                              // Transform::Rotate (Transform.cpp:214-228)
+      sincos_poly(angle, s, c);
+      const float omc = 1 - c;
+      const float m00 = c + x * x * omc, m01 = x * y * omc - z * s, m02 = x * z * omc + y * s;
+      const float m10 = y * x * omc + z * s, m11 = c + y * y * omc, m12 = y * z * omc - x * s;
+      const float m20 = z * x * omc - y * s, m21 = z * y * omc + x * s, m22 = c + z * z * omc;
+      const float rx = hx * m00 + hy * m01 + hz * m02;        // Transform::Apply(Vector)
+      const float ry = hx * m10 + hy * m11 + hz * m12;
+      const float rz = hx * m20 + hy * m21 + hz * m22;
+      const float k = 1 - rough;
+      const float ox = sx * k + rx * rough, oy = sy * k + ry * rough, oz = sz * k + rz * rough;
+      if (oy < 0) continue;
+      lx = ox; ly = oy; lz = oz;
+      return true;
+   }
+   return false;
+}
+
 // The bounce (PathTraceIntegrator.cpp:71-95): local incoming direction,
-// BRDF sample, world outgoing direction; updates src.
+// BRDF sample, world outgoing direction; updates src.  For glossy meshes
+// M.refl is the lobe's constant reflectance and M.emission[0] its roughness.
+template <bool kGlossy = true>
 __device__ __forceinline__ void bounce(const MeshRec &M, const Frame &fr, const float lin[3], uint32_t pixel,
                                        uint32_t sample, uint32_t bounce_index, uint32_t seed, float wout[3],
                                        float src[3]) {
    float lx, ly, lz;
+   bool live = true;
    if (M.brdf == PTX_BRDF_MIRROR) {            // MirrorBRDF::sample (mirror_brdf.h:23-35)
       lx = lin[0]; ly = -lin[1]; lz = lin[2];
+   } else if (kGlossy && M.brdf == PTX_BRDF_GLOSSY) {
+      live = glossy_sample(lin, M.emission[0], pixel, sample, bounce_index, seed, lx, ly, lz);
+      if (!live) { lx = 0.f; ly = 1.f; lz = 0.f; }
    } else {                                     // LambertBRDF -> AbstractBRDF::sample -> CosineSampleHemisphere
       uint32_t r0, r1;
       philox4x32_10(pixel, sample, bounce_index, 0u, seed, kRngKeyHi, r0, r1);
@@ -152,9 +230,9 @@ __device__ __forceinline__ void bounce(const MeshRec &M, const Frame &fr, const 
    float wz = fr.t1z * lx + fr.nz * ly + fr.t2z * lz;
    normalize3(wx, wy, wz);
    wout[0] = wx; wout[1] = wy; wout[2] = wz;
-   src[0] = src[0] * M.refl[0];
-   src[1] = src[1] * M.refl[1];
-   src[2] = src[2] * M.refl[2];
+   src[0] = live ? src[0] * M.refl[0] : 0.f;
+   src[1] = live ? src[1] * M.refl[1] : 0.f;
+   src[2] = live ? src[2] * M.refl[2] : 0.f;
 }
 
 // Intersection::WorldToLocal (Intersection.cpp:10-14)
@@ -596,7 +674,7 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
          }
          if (b >= J.max_depth) break;
          const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
-         if (nee && M.brdf != PTX_BRDF_MIRROR) {
+         if (nee && M.brdf == PTX_BRDF_LAMBERT) {
             for (uint32_t j = 0; j < S.num_lights; j++) {
                const float *light = S.lights + 6 * (size_t)j;
                float l[3], dist, g;
@@ -704,7 +782,7 @@ struct NeeRecord {
    float4 a, b, c;   // a = P.xyz, wout.x; b = wout.yz, k.xy; c = k.z, n.xyz
 };
 
-template <int kLds, bool kNee>
+template <int kLds, bool kNee, bool kGlossy>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
         uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
@@ -904,6 +982,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             M.refl[0] = __uint_as_float(mw[1]);
             M.refl[1] = __uint_as_float(mw[2]);
             M.refl[2] = __uint_as_float(mw[3]);
+            if (kGlossy) M.emission[0] = __uint_as_float(mw[5]);       // glossy roughness
             Frame fr;
             fr.lx = f.q[1]; fr.ly = f.q[2]; fr.lz = f.q[3];
             fr.t1x = f.q[4]; fr.t1y = f.q[5]; fr.t1z = f.q[6];
@@ -913,11 +992,11 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             src[0] = src[1] = src[2] = 1.f;
             const uint32_t x = pixel % c.width, y = pixel / c.width;
             const uint32_t dbg = (x == 128 && y == 128) ? kPDbg : 0u;
-            const bool nee = kNee && (mode & kPNee) && M.brdf != PTX_BRDF_MIRROR && S.num_lights > 0;
+            const bool nee = kNee && (mode & kPNee) && M.brdf == PTX_BRDF_LAMBERT && S.num_lights > 0;
             const float kd[3] = {M.refl[0], M.refl[1], M.refl[2]};
             const float s_before[3] = {src[0], src[1], src[2]};
             float wout[3];
-            bounce(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
+            bounce<kGlossy>(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
             nodes = S.nodes;
             tris = S.tris;
             if (nee) {
@@ -946,6 +1025,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const int32_t brdf = __float_as_int(f.q[0]);
             const int32_t is_light = __float_as_int(f.q[4]);
             const uint32_t has_normals = __float_as_uint(f.q[8]);
+            if (kGlossy) s.cnt = __float_as_uint(f.q[5]);          // glossy roughness, kept until the bounce
             if (is_light) {
                write_res(res, slot, fin<kNee>(Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
                                                        (f.q[7] + 0.f) * src[2]}, direct, mode));
@@ -971,17 +1051,18 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             MeshRec M;
             M.brdf = (int32_t)t0;
             M.refl[0] = t1; M.refl[1] = t2; M.refl[2] = t3;
+            if (kGlossy) M.emission[0] = __uint_as_float(s.cnt);
             Frame fr;
             hit_frame_sel(&f.q[0], &f.q[9], m == kPShade3, s.r, s.hu, s.hv, s.hw, s.after, fr);
             const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
             const uint32_t local = slot - c.base;
             const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
             const DevScene &S = ssc[c.scene];
-            const bool nee = kNee && (mode & kPNee) && M.brdf != PTX_BRDF_MIRROR && S.num_lights > 0;
+            const bool nee = kNee && (mode & kPNee) && M.brdf == PTX_BRDF_LAMBERT && S.num_lights > 0;
             const float s_before[3] = {src[0], src[1], src[2]};
             float lin[3], wout[3];
             world_to_local(fr, d, lin);
-            bounce(M, fr, lin, pixel, sample, b, c.seed, wout, src);
+            bounce<kGlossy>(M, fr, lin, pixel, sample, b, c.seed, wout, src);
             if (nee) {
                const float keep[3] = {src[0], src[1], src[2]};
                src[0] = s_before[0]; src[1] = s_before[1]; src[2] = s_before[2];
@@ -1063,11 +1144,18 @@ void launch_zero_floats(float *a, size_t n, hipStream_t st) {
 }
 
 int paths_blocks_per_cu() {
-   int a = 0, b = 0;
-   const hipError_t ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_paths<kRenderLds, false>, 256, 0);
-   const hipError_t eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_paths<kRenderLds, true>, 256, 0);
-   if (ea != hipSuccess || eb != hipSuccess) return 1;
-   return std::max(1, std::min(a, b));   // one grid and one scratch area serve both variants
+   int b[4] = {0, 0, 0, 0};
+   hipError_t e[4];
+   e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[0], dev::k_paths<kRenderLds, false, false>, 256, 0);
+   e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[1], dev::k_paths<kRenderLds, true, false>, 256, 0);
+   e[2] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[2], dev::k_paths<kRenderLds, false, true>, 256, 0);
+   e[3] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[3], dev::k_paths<kRenderLds, true, true>, 256, 0);
+   int m = b[0];
+   for (int i = 0; i < 4; i++) {
+      if (e[i] != hipSuccess) return 1;
+      m = std::min(m, b[i]);
+   }
+   return std::max(1, m);   // one grid and one scratch area serve every variant
 }
 
 uint32_t paths_max_chunks() { return kMaxRoundChunks; }
@@ -1075,14 +1163,16 @@ uint32_t paths_max_scenes() { return dev::kMaxRoundScenes; }
 
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-                  uint32_t leaf_min, float4 *nee_scratch, uint32_t grid, hipStream_t st) {
+                  uint32_t leaf_min, float4 *nee_scratch, bool glossy, uint32_t grid, hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
-   if (nee_scratch)
-      hipLaunchKernelGGL((dev::k_paths<kRenderLds, true>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs,
-                         scenes, nscenes, res, work, spill, max_stack, shade_min, leaf_min, nee_scratch);
-   else
-      hipLaunchKernelGGL((dev::k_paths<kRenderLds, false>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs,
-                         scenes, nscenes, res, work, spill, max_stack, shade_min, leaf_min, nee_scratch);
+#define PTX_LAUNCH_PATHS(N, G)                                                                                     \
+   hipLaunchKernelGGL((dev::k_paths<kRenderLds, N, G>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, \
+                      scenes, nscenes, res, work, spill, max_stack, shade_min, leaf_min, nee_scratch)
+   if (nee_scratch && glossy) PTX_LAUNCH_PATHS(true, true);
+   else if (nee_scratch) PTX_LAUNCH_PATHS(true, false);
+   else if (glossy) PTX_LAUNCH_PATHS(false, true);
+   else PTX_LAUNCH_PATHS(false, false);
+#undef PTX_LAUNCH_PATHS
 }
 
 void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {

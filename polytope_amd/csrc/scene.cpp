@@ -75,6 +75,13 @@ void world_vertices(const ptx_mesh_desc &m, WorldMesh &out) {
    std::copy(m.refl, m.refl + 3, out.rec.refl);
    out.rec.is_light = m.is_light;
    std::copy(m.emission, m.emission + 3, out.rec.emission);
+   if (m.brdf == PTX_BRDF_GLOSSY && !m.is_light) {
+      // GlossyBRDF::sample's reflectance Ks (1 - r) + Kd r (glossy_brdf.h:52) is
+      // constant per mesh; the roughness rides in the unused emission slot
+      const float r = m.roughness;
+      for (int c = 0; c < 3; c++) out.rec.refl[c] = m.specular[c] * (1 - r) + m.refl[c] * r;
+      out.rec.emission[0] = r;
+   }
    out.rec.has_normals = out.has_normals ? 1u : 0u;
 }
 

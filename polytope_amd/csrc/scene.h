@@ -72,4 +72,5 @@ struct ptx_scene {
    std::vector<void *> allocations;
    ptx_scene_info info{};
    uint32_t num_meshes = 0;
+   bool has_glossy = false;          // a mesh that is not a light has PTX_BRDF_GLOSSY
 };

@@ -42,9 +42,9 @@ The reference's quirks are kept, because they decide which scene it renders:
   Here such lights are collected separately for the library's opt-in
   next-event estimation; they do not change anything else.
 
-Not supported, with an error: textures, the "plastic" material (a glossy BRDF
-the GPU path does not have), shapes without a material, and anything the
-reference itself would stop on.
+The "plastic" material is the reference's GlossyBRDF (Kd, Ks, roughness).
+Not supported, with an error: textures, shapes without a material, and
+anything the reference itself would stop on.
 """
 from __future__ import annotations
 
@@ -59,7 +59,7 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from .scenes import LAMBERT, MIRROR, Camera, MeshData
+from .scenes import GLOSSY, LAMBERT, MIRROR, Camera, MeshData
 
 f32 = np.float32
 
@@ -693,10 +693,12 @@ def sphere_geometry(meridians: int, parallels: int) -> Geometry:
 
 @dataclass(eq=False)
 class _Material:
-    brdf: int                     # LAMBERT, MIRROR, or -1: none the GPU path can render
+    brdf: int                     # LAMBERT, MIRROR, GLOSSY, or -1: none the GPU path can render
     refl: Tuple[float, float, float]
     name: str = ""
     why: str = ""
+    specular: Tuple[float, float, float] = (0.25, 0.25, 0.25)
+    roughness: float = 0.1
 
 
 @dataclass(eq=False)
@@ -993,6 +995,9 @@ def _parse(directives: List[Directive], base: str, fname: str, strict: bool) -> 
         else:
             md.brdf = m.material.brdf
             md.refl = m.material.refl
+            if md.brdf == GLOSSY:
+                md.specular = m.material.specular
+                md.roughness = m.material.roughness
         out.append(md)
     if strict and unrenderable:
         i, why = unrenderable[0]
@@ -1020,7 +1025,18 @@ def _material(d: Directive) -> Optional[_Material]:
                 return _Material(MIRROR, tuple(float(x) for x in a.floats[:3]))
         return _Material(-1, (0.0, 0.0, 0.0), why="Material mirror without rgb Kr makes no material in the reference")
     if d.type == "plastic":
-        return _Material(-1, (0.0, 0.0, 0.0), why="the plastic material's glossy BRDF is not supported by the GPU path")
+        # pbrt_parser.cpp:791-842: Kd and Ks default to 0.25, roughness to 0.1;
+        # only rgb Kd / Ks and a float roughness are read
+        kd, ks, rough = [f32(0.25)] * 3, [f32(0.25)] * 3, f32(0.1)
+        for a in d.args:
+            if a.name == "Kd" and a.type == "rgb":
+                kd = a.floats[:3]
+            elif a.name == "Ks" and a.type == "rgb":
+                ks = a.floats[:3]
+            elif a.name == "roughness" and a.type == "float":
+                rough = a.floats[0]
+        return _Material(GLOSSY, tuple(float(x) for x in kd), specular=tuple(float(x) for x in ks),
+                         roughness=float(rough))
     return None
 
 

@@ -21,6 +21,7 @@ import numpy as np
 
 LAMBERT = 1
 MIRROR = 2
+GLOSSY = 3        # the reference's "plastic" material (GlossyBRDF); refl holds Kd
 
 IDENTITY = np.eye(4, dtype=np.float32)
 
@@ -35,6 +36,8 @@ class MeshData:
     refl: Tuple[float, float, float] = (0.5, 0.5, 0.5)
     is_light: bool = False
     emission: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    specular: Tuple[float, float, float] = (0.25, 0.25, 0.25)   # glossy: Ks
+    roughness: float = 0.1                                        # glossy roughness
     w2o: Optional[np.ndarray] = None       # world-to-object if given (the inverse of object_to_world otherwise)
 
     @property
@@ -465,6 +468,10 @@ def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov:
             lines.append("  Scale " + " ".join(_fmt(x) for x in s))
         if m.brdf == MIRROR:
             lines.append('  Material "mirror" "rgb Kr" [ %s ]' % " ".join(_fmt(x) for x in m.refl))
+        elif m.brdf == GLOSSY:
+            lines.append('  Material "plastic" "rgb Kd" [ %s ] "rgb Ks" [ %s ] "float roughness" [ %s ]'
+                         % (" ".join(_fmt(x) for x in m.refl), " ".join(_fmt(x) for x in m.specular),
+                            _fmt(m.roughness)))
         else:
             lines.append('  Material "matte" "rgb Kd" [ %s ]' % " ".join(_fmt(x) for x in m.refl))
         emission = m.emission
@@ -475,7 +482,8 @@ def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov:
         lines.append('  Shape "plymesh" "string filename" [ "%s" ]' % ply)
         lines.append("AttributeEnd")
         out_meshes.append(MeshData(v, fc, object_to_world=o2w, brdf=m.brdf,
-                                   refl=tuple(float(f(x)) for x in m.refl), is_light=m.is_light, emission=emission))
+                                   refl=tuple(float(f(x)) for x in m.refl), is_light=m.is_light, emission=emission,
+                                   specular=tuple(float(f(x)) for x in m.specular), roughness=float(f(m.roughness))))
     lines.append("WorldEnd")
     path = os.path.join(directory, name + ".pbrt")
     with open(path, "w") as fh:

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header():
-    assert C.sizeof(_abi.MeshDesc) == 240
+    assert C.sizeof(_abi.MeshDesc) == 256          # ... emission[3], specular[3], roughness
     assert C.sizeof(_abi.BvhNode) == 32
     assert C.sizeof(_abi.CameraDesc) == 16 * 4 + 4 + 8 + 16
     assert _abi.HIT_DTYPE.itemsize == 24

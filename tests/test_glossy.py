@@ -1,0 +1,89 @@
+"""The glossy ("plastic") lobe, GlossyBRDF::sample (glossy_brdf.h:27-60).
+
+Pinned three ways: against the reference's own function on the randomness it
+actually drew (its generator seeded and replayed, see the reference driver's
+--glossy-cases), with libm's acos/sin/cos as the reference calls them; the
+project-owned acos that CPU and GPU use instead, against the exact function;
+and closed forms (roughness 0 gives the mirror direction, the reflectance mix)."""
+import math
+import struct
+
+import numpy as np
+import pytest
+
+import oracle as O
+from polytope_amd import scenes as S
+
+needs_ref = pytest.mark.skipif(not O.reference_available(), reason="reference build (oracle/_ref) not present")
+
+
+def _cases(n, seed):
+    """Local incoming directions of rays arriving at a surface (y < 0: the frame's
+    normal faces the ray), with roughness and a seed.  An upward incoming ray
+    would make the reference's loop run forever at roughness 0."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        v = rng.normal(size=3)
+        v[1] = -abs(v[1]) - 1e-3
+        if i % 5 == 0:
+            v[1] *= 1e-2                 # grazing: the mirror direction is close to the surface
+        v = (v / np.linalg.norm(v)).astype(np.float32)
+        rough = np.float32([0.0, 0.1, 0.3, 0.5, 0.75, 0.9, 1.0][i % 7])
+        out.append((v, rough, int(rng.integers(1, 2 ** 31))))
+    return out
+
+
+@needs_ref
+def test_glossy_lobe_matches_reference_on_its_own_draws(tmp_path):
+    cases = _cases(300, 11)
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(struct.pack("<I", len(cases)))
+        for v, r, s in cases:
+            fh.write(struct.pack("<4fI", float(v[0]), float(v[1]), float(v[2]), float(r), s))
+    O.run_reference(["--glossy-cases", "in.bin", "out.bin"], cwd=str(tmp_path))
+    res = np.fromfile(tmp_path / "out.bin", np.float32).reshape(len(cases), 132)
+    multi = 0
+    for (v, r, _), row in zip(cases, res):
+        n = int(row[3:4].view(np.uint32)[0])
+        assert n >= 2 and n % 2 == 0 and n <= 128, n          # the draws were seen, two per attempt
+        got, used = O.glossy_sample(v, r, row[4:4 + n], libm=True)
+        assert got is not None and used == n // 2
+        assert np.array_equal(got.view(np.uint32), row[:3].view(np.uint32)), (v, r)
+        multi += used > 1
+    assert multi > 10                                          # rejections were exercised
+
+
+def test_project_acos_accuracy():
+    worst = 0
+    for x in np.concatenate([np.linspace(-1, 1, 20001, dtype=np.float32),
+                             np.array([-1, -0.5, -0.4999999, 0, 0.5, 0.50000006, 1], np.float32)]):
+        got = np.float32(O.acos(float(x)))
+        exact = math.acos(float(x))
+        ulp = np.spacing(np.float32(abs(exact))) if exact else np.float32(1e-45)
+        err = abs(float(got) - exact) / float(ulp)
+        worst = max(worst, err)
+    assert worst <= 2.0, worst
+    assert math.isnan(O.acos(1.0000001)) and math.isnan(O.acos(-1.5))
+
+
+def test_roughness_zero_is_the_mirror_direction():
+    for v, _, _ in _cases(50, 3):
+        got, used = O.glossy_sample(v, 0.0, [0.3, 0.7], libm=False)
+        m = np.array([v[0], -v[1], v[2]], np.float64)
+        assert used == 1 and np.allclose(got, m / np.linalg.norm(m), atol=3e-7)
+
+
+def test_glossy_reflectance_is_the_roughness_mix():
+    # GlossyBRDF::sample: refl = Ks (1 - r) + Kd r, constant per mesh (glossy_brdf.h:52)
+    v, f = S.quad(2.0, 0.0)
+    cam = S.Camera(S.look_at((0.0, 3.0, -0.5), (0.0, 0.0, 0.0)), 40.0, 8, 8)
+    for r, ks, kd in [(0.0, (0.9, 0.5, 0.25), (0.1, 0.2, 0.3)), (1.0, (0.9, 0.5, 0.25), (0.1, 0.2, 0.3))]:
+        m = S.MeshData(v, f, brdf=S.GLOSSY, refl=kd, specular=ks, roughness=r)
+        sky = (100.0, 100.0, 100.0)
+        film = O.OracleScene([m], sky).render(cam, 4, 1, 5)
+        want = ks if r == 0 else kd
+        # every path: one bounce off the quad, then the sky (the quad faces up)
+        vals = film.reshape(-1, 3)
+        for c in range(3):
+            assert np.allclose(vals[:, c], np.float32(100.0) * np.float32(want[c]), rtol=1e-6)

@@ -47,8 +47,9 @@ def _assert_same(ref, sf):
         if m.is_light:
             assert rm["spd"] == _bits(m.emission), i
         elif i not in unrenderable:
-            assert rm["brdf"] == {S.LAMBERT: "lambert", S.MIRROR: "mirror"}[m.brdf], i
-            assert rm["refl"] == _bits(m.refl), i
+            assert rm["brdf"] == {S.LAMBERT: "lambert", S.MIRROR: "mirror", S.GLOSSY: "glossy"}[m.brdf], i
+            if m.brdf != S.GLOSSY:
+                assert rm["refl"] == _bits(m.refl), i
         for j, axis in enumerate("xyz"):
             assert rm[axis] == _bits(m.verts[:, j]), (i, axis)
         faces = np.asarray(m.faces, np.uint32)
@@ -183,6 +184,7 @@ AttributeBegin
 AttributeEnd
 Shape "plymesh" "string filename" [ "ico.ply" ]
 Scale 0.5 0.5 0.5
+Material "plastic" "rgb Kd" [ .2 .4 .6 ] "rgb Ks" [ .7 .7 .7 ] "float roughness" [ .35 ]
 Shape "sphere" "float radius" [ 3 ]
 WorldEnd
 """,
@@ -286,11 +288,15 @@ def test_unrenderable_meshes_are_refused_when_strict(tmp_path):
     path = tmp_path / "s.pbrt"
     path.write_text('WorldBegin\nShape "plymesh" "string filename" [ "ico.ply" ]\n'
                     'Material "plastic" "rgb Kd" [ .5 .5 .5 ]\nShape "plymesh" "string filename" [ "ico.ply" ]\n'
+                    'Material "matte" "texture Kd" "checks"\nShape "plymesh" "string filename" [ "ico.ply" ]\n'
                     'WorldEnd\n')
     with pytest.raises(P.PbrtError, match="no material"):
         P.read_pbrt(str(path))
     sf = P.read_pbrt(str(path), strict=False)
-    assert [i for i, _ in sf.unrenderable] == [0, 1] and "glossy" in sf.unrenderable[1][1]
+    assert [i for i, _ in sf.unrenderable] == [0, 2] and "textured" in sf.unrenderable[1][1]
+    g = sf.meshes[1]                     # plastic: GlossyBRDF, Ks and roughness at their defaults
+    assert g.brdf == S.GLOSSY and g.refl == (0.5, 0.5, 0.5)
+    assert g.specular == tuple(float(np.float32(0.25)) for _ in range(3)) and g.roughness == float(np.float32(0.1))
 
 
 # ----------------------------------------------------------------------------- writer round trip
