@@ -69,7 +69,26 @@ typedef struct ptx_mesh_desc {
    float emission[3];                    /* *Mesh::spd, 0-255 units              */
    float specular[3];                    /* glossy: Ks                           */
    float roughness;                      /* glossy: 0 = mirror-like .. 1        */
+   /* Textured Lambert: LambertBRDF(texture) (lambert_brdf.h:31-56).  u, v are
+    * mesh_geometry::u_packed / v_packed (mesh.h:56-57), num_uv entries each,
+    * indexed by the face vertex indices.  texture = k > 0 takes the reflectance
+    * from ptx_scene_desc.textures[k - 1] instead of refl; 0 (a zeroed
+    * descriptor) = untextured, u and v ignored. */
+   const float *u, *v;
+   uint32_t num_uv;
+   uint32_t texture;
 } ptx_mesh_desc;
+
+/* An image texture, poly::texture (src/cpu/shading/texture.h:12-42) as the
+ * parser decodes a "color"/"spectrum" imagemap (pbrt_parser.cpp:730-776):
+ * RGBA8 rows top to bottom.  The texel of (u, v) is column (unsigned)(u * W),
+ * row (unsigned)((1 - v) * H), reflectance = byte * (1.0f / 255.0f) per RGB
+ * channel (spectrum.h:17-18).  Coordinates outside the image clamp to the
+ * nearest edge texel, where the reference reads outside its texel array. */
+typedef struct ptx_texture_desc {
+   uint32_t width, height;               /* 1 .. 32767 each                      */
+   const uint8_t *rgba;                  /* width * height * 4 bytes             */
+} ptx_texture_desc;
 
 /* poly::compact_bvh_node, byte-compatible (src/cpu/acceleration/bvh.h:24-53). */
 typedef struct ptx_bvh_node {
@@ -103,6 +122,8 @@ typedef struct ptx_scene_desc {
    uint32_t build_threads; /* 0 = all host cores */
    const ptx_point_light *lights;   /* optional, for jobs with PTX_JOB_NEE */
    uint32_t num_lights;
+   const ptx_texture_desc *textures;   /* ptx_mesh_desc.texture = k selects textures[k - 1] */
+   uint32_t num_textures;
 } ptx_scene_desc;
 
 typedef struct ptx_scene_info {

@@ -59,6 +59,12 @@ def lib() -> C.CDLL:
         L.orc_scene_create.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
         L.orc_scene_set_lights.restype = None
         L.orc_scene_set_lights.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_scene_set_textures.restype = None
+        L.orc_scene_set_textures.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_tex_lerp.restype = None
+        L.orc_tex_lerp.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_void_p]
+        L.orc_texel_index.restype = C.c_uint32
+        L.orc_texel_index.argtypes = [C.c_float, C.c_float, C.c_uint32, C.c_uint32]
         L.orc_scene_create_prebuilt.restype = C.c_void_p
         L.orc_scene_create_prebuilt.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_uint32,
                                                 C.c_void_p, C.c_uint32]
@@ -97,8 +103,9 @@ class OracleScene:
         """bvh: optional (nodes, pairs) in the reference compact layout to use
         instead of building the restated tree; lights: optional point lights
         [(position xyz, intensity rgb), ...] for next-event estimation."""
-        from polytope_amd import mesh_descs
+        from polytope_amd import mesh_descs, texture_descs
         arr, keep = mesh_descs(meshes)
+        tex, tex_keep = texture_descs(meshes)
         sky = (C.c_float * 3)(*(skybox if skybox is not None else (0.0, 0.0, 0.0)))
         if bvh is None:
             self.h = lib().orc_scene_create(C.cast(arr, C.c_void_p), len(meshes), 1 if skybox is not None else 0,
@@ -110,6 +117,8 @@ class OracleScene:
                                                      1 if skybox is not None else 0, C.cast(sky, C.c_void_p),
                                                      nodes.ctypes.data, len(nodes), pairs.ctypes.data, len(pairs))
         self._keep = (arr, keep)
+        if tex_keep:
+            lib().orc_scene_set_textures(self.h, C.cast(tex, C.c_void_p), len(tex_keep))
         if lights:
             flat = np.ascontiguousarray([[*p, *i] for p, i in lights], np.float32)
             lib().orc_scene_set_lights(self.h, flat.ctypes.data, len(flat))
@@ -182,6 +191,20 @@ def glossy_sample(incoming, roughness: float, uniforms, libm: bool):
     ok = lib().orc_glossy_sample(v.ctypes.data, C.c_float(roughness), u.ctypes.data, len(u) // 2, 1 if libm else 0,
                                  out.ctypes.data, C.byref(used))
     return (out if ok else None), used.value
+
+
+def tex_lerp(uv, u: float, v: float, w: float) -> Tuple[float, float]:
+    """Texture coordinates of a hit with barycentric weights u, v, w on a face
+    whose corners have coordinates uv = (u0, v0, u1, v1, u2, v2)."""
+    c = np.ascontiguousarray(uv, np.float32)
+    out = np.zeros(2, np.float32)
+    lib().orc_tex_lerp(c.ctypes.data, C.c_float(u), C.c_float(v), C.c_float(w), out.ctypes.data)
+    return float(out[0]), float(out[1])
+
+
+def texel_index(tu: float, tv: float, width: int, height: int) -> int:
+    """Flat texel index (row * width + column) the texture lookup reads."""
+    return int(lib().orc_texel_index(C.c_float(tu), C.c_float(tv), width, height))
 
 
 def box_hits(box, ray, min_t: float) -> bool:

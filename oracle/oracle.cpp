@@ -19,6 +19,8 @@
 //   local frame          Intersection::WorldToLocal/LocalToWorld src/cpu/structures/Intersection.cpp:10-23
 //   BRDFs                LambertBRDF/AbstractBRDF/MirrorBRDF::sample
 //                        src/cpu/shading/brdf/{lambert_brdf.h:48-57,abstract_brdf.h:36-45,mirror_brdf.h:23-35}
+//   textures             Mesh::intersect's u/v_tex_lerp (mesh.cpp:342-353), texture::evaluate_rgba
+//                        (src/cpu/shading/texture.h:25-31), ReflectanceSpectrum(int,int,int) (spectrum.h:17-18)
 //                        GlossyBRDF::sample (the "plastic" material)  src/cpu/shading/brdf/glossy_brdf.h:27-60
 //   box filter           BoxFilter::AddSample/Output            src/cpu/filters/BoxFilter.cpp:14-71
 //   runner               TileRunner::Run 2x2 tiles              src/cpu/runners/TileRunner.cpp:61-117
@@ -30,6 +32,9 @@
 //     seed, counter (pixel, sample, bounce, 0); u = (r >> 8) * 2^-24.
 //   * sin/cos in CosineSampleHemisphere (src/cpu/constants.h:106-115) use a
 //     project-owned polynomial (ptx_sincos below) so CPU and GPU agree bitwise.
+//   * Texels: where texture::evaluate_rgba's flat index falls outside the
+//     image, the reference reads outside its texel array; here the column and
+//     row clamp to the image (texel_index below).
 //   * Post-hit frame: the reference re-runs the post-hit block after every face
 //     test once a hit exists, with the mesh of the face just tested and
 //     re-adding the origin offset each time (mesh.cpp:357-462).  Here the frame
@@ -188,9 +193,22 @@ struct MeshIn {            // mirrors polytope_amd's ptx_mesh_desc (C layout, se
    float emission[3];
    float specular[3];                // glossy: Ks (refl holds Kd)
    float roughness;                  // glossy: 0 specular .. 1 diffuse
+   const float *u, *v;               // mesh_geometry::u_packed / v_packed (num_uv each), or null
+   uint32_t num_uv;
+   uint32_t texture;                 // textured Lambert: 1 + index into the scene's textures; 0: none
 };
 
-static_assert(sizeof(MeshIn) == 256, "MeshIn mirrors ptx_mesh_desc");
+static_assert(sizeof(MeshIn) == 280, "MeshIn mirrors ptx_mesh_desc");
+
+struct TexIn {                       // mirrors ptx_texture_desc
+   uint32_t width, height;
+   const uint8_t *rgba;              // rows top to bottom
+};
+
+struct Texture {                     // poly::texture (src/cpu/shading/texture.h:12-42), RGBA data
+   uint32_t w = 0, h = 0;
+   std::vector<uint8_t> rgba;
+};
 
 struct Mesh {
    uint32_t nf = 0;
@@ -202,6 +220,8 @@ struct Mesh {
    float rough = 0;                  // glossy roughness
    bool light = false;
    float L[3] = {0, 0, 0};
+   int tex = -1;                     // textured Lambert: texture index
+   std::vector<float> uv;            // textured: u, v of corners fv0, fv1, fv2 per face (6 floats)
 };
 
 struct CompactNode {      // compact_bvh_node (src/cpu/acceleration/bvh.h:24-53), 32 bytes
@@ -235,6 +255,7 @@ struct Scene {
    bool has_sky = false;
    float sky[3] = {0, 0, 0};
    std::vector<PointLight> lights;     // used only by next-event estimation
+   std::vector<Texture> textures;
 };
 
 // bvh::bound (src/cpu/acceleration/bvh.cpp:36-395).  The reference copies each
@@ -759,7 +780,19 @@ static bool glossy_sample(V3 in, float rough, Draw draw, V3& out, uint32_t* used
 
 // A mesh's shading data.  Glossy meshes keep the reflectance GlossyBRDF::sample
 // returns, Ks (1 - r) + Kd r per channel (glossy_brdf.h:52), which is constant.
+// A textured Lambert mesh keeps its faces' corner texture coordinates.
 static void set_material(Mesh& m, const MeshIn& in) {
+   if (in.texture > 0 && in.brdf == 1 && !in.is_light && in.u && in.v) {
+      m.tex = (int)in.texture - 1;
+      m.uv.resize(size_t(6) * in.num_faces);
+      for (uint32_t f = 0; f < in.num_faces; f++) {
+         const uint32_t ix[3] = {in.fv0[f], in.fv1[f], in.fv2[f]};
+         for (int c = 0; c < 3; c++) {
+            m.uv[6 * f + 2 * c] = in.u[ix[c]];
+            m.uv[6 * f + 2 * c + 1] = in.v[ix[c]];
+         }
+      }
+   }
    m.brdf = in.brdf;
    std::memcpy(m.refl, in.refl, sizeof(m.refl));
    if (in.brdf == 3) {
@@ -803,6 +836,45 @@ struct Camera {
 };
 
 struct Spd { float r, g, b; };
+
+// Mesh::intersect's interpolated texture coordinates (mesh.cpp:342-353): u, v,
+// w are the hit's barycentric weights of the face's corners 0, 1, 2.
+static inline void tex_lerp(const float uv[6], float u, float v, float w, float& tu, float& tv) {
+   tu = uv[0] * u + uv[2] * v + uv[4] * w;
+   tv = uv[1] * u + uv[3] * v + uv[5] * w;
+}
+
+// (unsigned int)f where C++ defines it (0 <= f < 2^32); 0 below, 0xFFFFFFFF above.
+static inline uint32_t texel_trunc(float f) {
+   if (!(f > 0.f)) return 0u;
+   if (!(f < 4294967296.f)) return 0xFFFFFFFFu;
+   return (uint32_t)f;
+}
+
+// texture::evaluate_rgba's texel (texture.h:25-31): flat index row * W + column
+// of column (unsigned)(u * W), row (unsigned)((1 - v) * H).  Where that index
+// is outside the image (the reference reads outside its data vector) the
+// column and row are clamped to the image instead.
+static inline uint32_t texel_index(float tu, float tv, uint32_t W, uint32_t H) {
+   const uint32_t col = texel_trunc(tu * (float)W);
+   const uint32_t row = texel_trunc((1.f - tv) * (float)H);
+   const uint64_t flat = (uint64_t)row * W + col;
+   if (flat < (uint64_t)W * H) return (uint32_t)flat;
+   return std::min(row, H - 1) * W + std::min(col, W - 1);
+}
+
+// LambertBRDF::sample's reflectance with a texture (lambert_brdf.h:49-56):
+// ReflectanceSpectrum(int, int, int) of the texel's R, G, B (spectrum.h:17-18).
+static inline void textured_refl(const Scene& s, const Mesh& m, const Isect& is, float refl[3]) {
+   float tu, tv;
+   tex_lerp(&m.uv[6 * is.face], is.u, is.v, is.w, tu, tv);
+   const Texture& t = s.textures[m.tex];
+   const uint8_t* px = &t.rgba[4 * size_t(texel_index(tu, tv, t.w, t.h))];
+   const float one_over_255 = 1.0f / 255.0f;
+   refl[0] = (float)px[0] * one_over_255;
+   refl[1] = (float)px[1] * one_over_255;
+   refl[2] = (float)px[2] * one_over_255;
+}
 
 // PathTraceIntegrator::GetSample (src/cpu/integrators/PathTraceIntegrator.cpp:10-135)
 // Next-event estimation at a Lambert vertex (opt-in; no reference
@@ -854,7 +926,9 @@ static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t
       const Mesh& m = s.meshes[is.mesh];
       if (m.light) return done({(m.L[0] + 0.f) * sr, (m.L[1] + 0.f) * sg, (m.L[2] + 0.f) * sb});
       if (bounces >= max_depth) return done({0, 0, 0});
-      if (nee && m.brdf == 1) add_direct(s, is, m.refl, sr, sg, sb, direct);
+      float kd[3] = {m.refl[0], m.refl[1], m.refl[2]};
+      if (m.tex >= 0) textured_refl(s, m, is, kd);
+      if (nee && m.brdf == 1) add_direct(s, is, kd, sr, sg, sb, direct);
       const uint32_t bounce_index = bounces;
       bounces++;
       // Intersection::WorldToLocal (Intersection.cpp:10-14); bent normal == geo normal
@@ -880,7 +954,7 @@ static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t
             continue;
          }
       } else {
-         rr = m.refl[0]; rg = m.refl[1]; rb = m.refl[2];
+         rr = kd[0]; rg = kd[1]; rb = kd[2];
          const uint32_t ctr[4] = {pixel_index, sample_index, bounce_index, 0u};
          const uint32_t key[2] = {seed, kRngKeyHi};
          uint32_t rnd[4];
@@ -986,6 +1060,24 @@ void orc_scene_set_lights(orc::Scene* s, const float* lights, uint32_t n) {
       s->lights[i] = {{lights[6 * i], lights[6 * i + 1], lights[6 * i + 2]},
                       {lights[6 * i + 3], lights[6 * i + 4], lights[6 * i + 5]}};
 }
+// Textures that meshes' MeshIn::texture indexes (copied).
+void orc_scene_set_textures(orc::Scene* s, const orc::TexIn* tex, uint32_t n) {
+   s->textures.resize(n);
+   for (uint32_t i = 0; i < n; i++) {
+      s->textures[i].w = tex[i].width;
+      s->textures[i].h = tex[i].height;
+      s->textures[i].rgba.assign(tex[i].rgba, tex[i].rgba + 4 * size_t(tex[i].width) * tex[i].height);
+   }
+}
+
+// The texture coordinates of a hit and the texel they select (for unit checks).
+void orc_tex_lerp(const float uv[6], float u, float v, float w, float out[2]) {
+   orc::tex_lerp(uv, u, v, w, out[0], out[1]);
+}
+uint32_t orc_texel_index(float tu, float tv, uint32_t width, uint32_t height) {
+   return orc::texel_index(tu, tv, width, height);
+}
+
 uint32_t orc_num_nodes(const orc::Scene* s) { return (uint32_t)s->nodes.size(); }
 uint32_t orc_num_pairs(const orc::Scene* s) { return (uint32_t)s->pairs.size(); }
 void orc_copy_bvh(const orc::Scene* s, void* nodes, uint32_t* pairs) {

@@ -43,6 +43,7 @@
 #include "cpu/shading/brdf/glossy_brdf.h"
 #include "cpu/shading/brdf/lambert_brdf.h"
 #include "cpu/shading/brdf/mirror_brdf.h"
+#include "cpu/shading/texture.h"
 #include "cpu/shapes/mesh.h"
 #include "cpu/structures/stats.h"
 #include "ptxm_scene.h"
@@ -118,7 +119,12 @@ void describe(const poly::AbstractRunner& runner, const std::string& path) {
       const poly::AbstractBRDF* brdf = m.material ? m.material->BRDF.get() : nullptr;
       const char* kind = "none";
       const poly::ReflectanceSpectrum* refl = nullptr;
-      if (const auto* lam = dynamic_cast<const poly::LambertBRDF*>(brdf)) { kind = "lambert"; refl = &lam->refl; }
+      const poly::texture* tex = nullptr;
+      if (const auto* lam = dynamic_cast<const poly::LambertBRDF*>(brdf)) {
+         kind = "lambert";
+         if (lam->texture) tex = lam->texture.get();
+         else refl = &lam->refl;
+      }
       else if (const auto* mir = dynamic_cast<const poly::MirrorBRDF*>(brdf)) { kind = "mirror"; refl = &mir->refl; }
       else if (dynamic_cast<const poly::GlossyBRDF*>(brdf)) kind = "glossy";     // its parameters are private
       else if (brdf) kind = "other";
@@ -128,6 +134,20 @@ void describe(const poly::AbstractRunner& runner, const std::string& path) {
          std::fprintf(f, ", \"refl\": ");
          farr(r, 3);
       }
+      if (tex) {
+         // FNV-1a 64 of the decoded texel bytes
+         uint64_t h = 1469598103934665603ull;
+         for (unsigned char c : tex->data) { h ^= c; h *= 1099511628211ull; }
+         std::fprintf(f, ", \"texture\": {\"name\": \"%s\", \"width\": %u, \"height\": %u, \"format\": \"%s\", "
+                         "\"bytes\": %zu, \"fnv1a\": \"%016llx\"}",
+                      tex->name.c_str(), tex->width, tex->height,
+                      tex->data_format == poly::texture::format::RGBA ? "rgba" : "grey", tex->data.size(),
+                      (unsigned long long)h);
+      }
+      std::fprintf(f, ", \"has_uv\": %s, \"u\": ", g.has_vertex_uvs ? "true" : "false");
+      farr(g.u_packed.data(), g.u_packed.size());
+      std::fprintf(f, ", \"v\": ");
+      farr(g.v_packed.data(), g.v_packed.size());
       std::fprintf(f, ", \"has_normals\": %s, \"x\": ", g.has_vertex_normals ? "true" : "false");
       farr(g.x_packed.data(), g.x_packed.size());
       std::fprintf(f, ", \"y\": ");
@@ -188,10 +208,62 @@ void glossy_cases(const std::string& in_path, const std::string& out_path) {
    write_file(out_path, out.data(), out.size() * 4);
 }
 
+// Texture lookups on the reference's own path: for each ray, Scene::intersect,
+// then, when the hit mesh is a textured Lambert, LambertBRDF::sample with the
+// intersection's u/v_tex_lerp (PathTraceIntegrator.cpp:78, lambert_brdf.h:49-56)
+// for the reflectance.  That call is made only where texture::evaluate_rgba's
+// index lies inside the texel array (texture.h:25-31); elsewhere the reference
+// reads outside it and the record says so.  Record per ray (36 bytes):
+//   flags (1 hit, 2 mesh has uvs, 4 textured Lambert, 8 index inside, 16
+//   column/row conversion defined), mesh, face, u_tex, v_tex, index, r, g, b.
+struct TexProbe {
+   uint32_t flags, mesh, face;
+   float tu, tv;
+   uint32_t index;
+   float r, g, b;
+};
+static_assert(sizeof(TexProbe) == 36, "tex probe record");
+
+void tex_probe(poly::Scene* scene, const std::vector<float>& rays, const std::string& out_path) {
+   const size_t n = rays.size() / 6;
+   std::vector<TexProbe> out(n);
+   for (size_t i = 0; i < n; i++) {
+      poly::Ray ray(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2], rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+      poly::Intersection is = scene->intersect(ray, -1, -1);
+      TexProbe& p = out[i];
+      std::memset(&p, 0, sizeof(p));
+      if (!is.Hits) { p.mesh = p.face = 0xFFFFFFFFu; continue; }
+      p.flags = 1;
+      uint32_t mi = 0;
+      for (; mi < scene->Shapes.size(); mi++) if (scene->Shapes[mi] == is.Shape) break;
+      p.mesh = mi; p.face = is.faceIndex;
+      if (!is.Shape->mesh_geometry->has_vertex_uvs) continue;
+      p.flags |= 2;
+      p.tu = is.u_tex_lerp; p.tv = is.v_tex_lerp;
+      const poly::AbstractBRDF* brdf = is.Shape->material ? is.Shape->material->BRDF.get() : nullptr;
+      const auto* lam = dynamic_cast<const poly::LambertBRDF*>(brdf);
+      if (!lam || !lam->texture) continue;
+      p.flags |= 4;
+      const poly::texture& t = *lam->texture;
+      const float fu = p.tu * (float)t.width, fv = (1.f - p.tv) * (float)t.height;
+      if (!(fu >= 0.f && fu < 4294967296.f && fv >= 0.f && fv < 4294967296.f)) continue;
+      p.flags |= 16;
+      const unsigned int u_texel = (unsigned int)fu, v_texel = (unsigned int)fv;
+      p.index = v_texel * t.width + u_texel;
+      if (size_t(p.index) * 4 + 2 >= t.data.size()) continue;
+      p.flags |= 8;
+      poly::ReflectanceSpectrum refl;
+      float pdf = 0;
+      (void)lam->sample(poly::Vector(0.f, -1.f, 0.f), p.tu, p.tv, refl, pdf);
+      p.r = refl.r; p.g = refl.g; p.b = refl.b;
+   }
+   write_file(out_path, out.data(), out.size() * sizeof(TexProbe));
+}
+
 void usage() {
    std::fprintf(stderr,
       "usage: ref_driver (--ptxm scene.bin | --pbrt scene.pbrt) [--nodes out] [--pairs out] [--describe out.json]\n"
-      "                  [--rays rays.bin --hits out] [--render out] [--threads N]\n"
+      "                  [--rays rays.bin --hits out] [--rays rays.bin --tex-probe out] [--render out] [--threads N]\n"
       "                  [--time-rays rays.bin --threads N --repeat R] [--time-render --threads N]\n");
    std::exit(2);
 }
@@ -200,7 +272,7 @@ void usage() {
 
 int main(int argc, char** argv) {
    std::string ptxm, pbrt, nodes_out, pairs_out, rays_in, hits_out, render_out, time_rays, describe_out;
-   std::string glossy_in, glossy_out;
+   std::string glossy_in, glossy_out, tex_out;
    bool png = false;
    bool time_render = false;
    unsigned threads = 1, repeat = 1;
@@ -213,6 +285,7 @@ int main(int argc, char** argv) {
       else if (a == "--pairs") pairs_out = next();
       else if (a == "--rays") rays_in = next();
       else if (a == "--hits") hits_out = next();
+      else if (a == "--tex-probe") tex_out = next();
       else if (a == "--render") render_out = next();
       else if (a == "--time-rays") time_rays = next();
       else if (a == "--threads") threads = (unsigned)std::atoi(next().c_str());
@@ -302,6 +375,8 @@ int main(int argc, char** argv) {
       }
       write_file(hits_out, out.data(), out.size() * sizeof(HitRecord));
    }
+
+   if (!rays_in.empty() && !tex_out.empty()) tex_probe(scene, load_rays(rays_in), tex_out);
 
    if (!time_rays.empty()) {
       // CPU baseline: contiguous chunks of the ray list per thread, as a ray-batch

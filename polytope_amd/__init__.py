@@ -3,7 +3,8 @@
 Python face of libptx_hip.so (C ABI in include/ptx.h).  The host-side objects
 mirror the reference's scene API: a :class:`Scene` is built from meshes
 (``mesh_geometry`` + ``Mesh``: packed vertices, faces, object-to-world
-transform, Lambert/mirror material, optional emission) and an optional
+transform, Lambert (constant or textured), mirror or glossy material,
+optional emission) and an optional
 constant skybox; its BVH is the reference's binned-SAH tree
 (src/cpu/acceleration/bvh.cpp), built on the host.  Queries and renders run
 on the GPU only: there is no CPU fallback, and a missing library or device
@@ -18,11 +19,11 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BVH_NODE_DTYPE, COUNTER_DTYPE, FRAME_DTYPE, HIT_DTYPE, PtxError, check, load)
-from .scenes import Camera, MeshData
+from .scenes import Camera, MeshData, Texture
 from .pbrt import PbrtError, SceneFile, read_pbrt, read_pbrt_string
 
 __all__ = ["Device", "Scene", "PtxError", "render", "render_batch", "Server", "autotune", "HIT_DTYPE", "COUNTER_DTYPE", "FRAME_DTYPE", "BVH_NODE_DTYPE",
-           "mesh_descs", "build_bvh", "camera_desc", "job_desc", "MeshData", "Camera"]
+           "mesh_descs", "texture_descs", "build_bvh", "camera_desc", "job_desc", "MeshData", "Camera", "Texture"]
 
 
 def _ptr(a: Optional[np.ndarray], ctype):
@@ -31,10 +32,38 @@ def _ptr(a: Optional[np.ndarray], ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+def texture_table(meshes: Sequence[MeshData]) -> List[Texture]:
+    """The distinct textures of ``meshes`` in order of first use: the scene's
+    texture table: ptx_mesh_desc.texture = k selects entry k - 1."""
+    out: List[Texture] = []
+    seen = {}
+    for m in meshes:
+        t = getattr(m, "texture", None)
+        if t is not None and id(t) not in seen:
+            seen[id(t)] = len(out)
+            out.append(t)
+    return out
+
+
+def texture_descs(meshes: Sequence[MeshData]) -> Tuple[C.Array, list]:
+    """The ptx_texture_desc array of ``texture_table(meshes)`` with the buffers it points into."""
+    table = texture_table(meshes)
+    arr = (_abi.TextureDesc * max(1, len(table)))()
+    keep = []
+    for i, t in enumerate(table):
+        a = np.ascontiguousarray(t.rgba, np.uint8)
+        keep.append(a)
+        arr[i].width, arr[i].height = a.shape[1], a.shape[0]
+        arr[i].rgba = a.ctypes.data_as(C.POINTER(C.c_uint8))
+    return arr, keep
+
+
 def mesh_descs(meshes: Sequence[MeshData]) -> Tuple[C.Array, list]:
-    """Builds the ptx_mesh_desc array; returns it with the numpy buffers it points into."""
+    """Builds the ptx_mesh_desc array; returns it with the numpy buffers it points
+    into.  A textured mesh's ``texture`` field indexes ``texture_table(meshes)``."""
     keep = []
     arr = (_abi.MeshDesc * len(meshes))()
+    tex_index = {id(t): i for i, t in enumerate(texture_table(meshes))}
     for i, m in enumerate(meshes):
         v = np.ascontiguousarray(m.verts, np.float32)
         f = np.ascontiguousarray(m.faces, np.uint32)
@@ -59,6 +88,14 @@ def mesh_descs(meshes: Sequence[MeshData]) -> Tuple[C.Array, list]:
         d.emission[:] = [float(x) for x in m.emission]
         d.specular[:] = [float(x) for x in m.specular]
         d.roughness = float(m.roughness)
+        if getattr(m, "texture", None) is not None:
+            d.texture = tex_index[id(m.texture)] + 1           # 0: untextured
+        if getattr(m, "uv", None) is not None:
+            uv = np.ascontiguousarray(m.uv, np.float32).reshape(-1, 2)
+            us, vs = np.ascontiguousarray(uv[:, 0]), np.ascontiguousarray(uv[:, 1])
+            keep += [us, vs]
+            d.u, d.v = _ptr(us, C.c_float), _ptr(vs, C.c_float)
+            d.num_uv = uv.shape[0]
     return arr, keep
 
 
@@ -151,9 +188,14 @@ class Scene:
         self.device = device
         self.meshes = list(meshes)
         arr, keep = mesh_descs(self.meshes)
+        tex, tex_keep = texture_descs(self.meshes)
+        keep += tex_keep
         desc = _abi.SceneDesc()
         desc.meshes = C.cast(arr, C.POINTER(_abi.MeshDesc))
         desc.num_meshes = len(self.meshes)
+        if tex_keep:
+            desc.textures = C.cast(tex, C.POINTER(_abi.TextureDesc))
+            desc.num_textures = len(tex_keep)
         if bvh is not None:
             nodes = np.ascontiguousarray(bvh[0]).view(BVH_NODE_DTYPE)
             pairs = np.ascontiguousarray(bvh[1], np.uint32).reshape(-1)

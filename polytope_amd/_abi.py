@@ -36,7 +36,12 @@ class MeshDesc(C.Structure):
                 ("object_to_world", C.c_float * 16), ("world_to_object", C.c_float * 16),
                 ("brdf", C.c_int32), ("refl", C.c_float * 3),
                 ("is_light", C.c_int32), ("emission", C.c_float * 3),
-                ("specular", C.c_float * 3), ("roughness", C.c_float)]
+                ("specular", C.c_float * 3), ("roughness", C.c_float),
+                ("u", _fp), ("v", _fp), ("num_uv", C.c_uint32), ("texture", C.c_uint32)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.POINTER(C.c_uint8))]
 
 
 class BvhNode(C.Structure):
@@ -53,7 +58,8 @@ class SceneDesc(C.Structure):
                 ("nodes", C.POINTER(BvhNode)), ("num_nodes", C.c_uint32),
                 ("leaf_pairs", _up), ("num_pairs", C.c_uint32),
                 ("has_skybox", C.c_int32), ("skybox", C.c_float * 3),
-                ("build_threads", C.c_uint32), ("lights", C.POINTER(C.c_float)), ("num_lights", C.c_uint32)]
+                ("build_threads", C.c_uint32), ("lights", C.POINTER(C.c_float)), ("num_lights", C.c_uint32),
+                ("textures", C.POINTER(TextureDesc)), ("num_textures", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):

@@ -3,6 +3,7 @@
 // to the runner's film.
 #include "GpuRunner.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -14,6 +15,7 @@
 #include "cpu/scenes/skyboxes/ColorSkybox.h"
 #include "cpu/shading/brdf/lambert_brdf.h"
 #include "cpu/shading/brdf/mirror_brdf.h"
+#include "cpu/shading/texture.h"
 #include "cpu/shapes/mesh.h"
 
 namespace poly {
@@ -65,6 +67,8 @@ void GpuRunner::Run(int threadId) {
 
    // meshes: packed geometry, transforms, material, emission (mesh.h:17-113)
    std::vector<ptx_mesh_desc> meshes(Scene->Shapes.size());
+   std::vector<ptx_texture_desc> textures;          // one per distinct poly::texture
+   std::vector<const texture *> texture_of;
    for (size_t i = 0; i < Scene->Shapes.size(); i++) {
       const Mesh &m = *Scene->Shapes[i];
       const mesh_geometry &g = *m.mesh_geometry;
@@ -87,12 +91,29 @@ void GpuRunner::Run(int threadId) {
       copy_matrix(*m.world_to_object, d.world_to_object);
       const AbstractBRDF *brdf = m.material ? m.material->BRDF.get() : nullptr;
       if (const auto *lam = dynamic_cast<const LambertBRDF *>(brdf)) {
-         if (lam->texture) {
-            fail(PTX_ERR_INVALID, "GpuRunner: textured Lambert materials are not supported");
-            return;
-         }
          d.brdf = PTX_BRDF_LAMBERT;
          d.refl[0] = lam->refl.r; d.refl[1] = lam->refl.g; d.refl[2] = lam->refl.b;
+         if (lam->texture && !m.spd) {      // LambertBRDF(texture): texels and u/v_packed
+            const texture *t = lam->texture.get();
+            if (t->data_format != texture::format::RGBA) {
+               fail(PTX_ERR_INVALID, "GpuRunner: mesh " + std::to_string(i) + " has a grey texture");
+               return;
+            }
+            if (!g.has_vertex_uvs || g.u_packed.empty()) {
+               fail(PTX_ERR_INVALID, "GpuRunner: mesh " + std::to_string(i) + " has a texture but no u, v");
+               return;
+            }
+            size_t k = 0;
+            while (k < texture_of.size() && texture_of[k] != t) k++;
+            if (k == texture_of.size()) {
+               texture_of.push_back(t);
+               textures.push_back(ptx_texture_desc{t->width, t->height, t->data.data()});
+            }
+            d.texture = (uint32_t)k + 1;
+            d.u = g.u_packed.data();
+            d.v = g.v_packed.data();
+            d.num_uv = (uint32_t)std::min(g.u_packed.size(), g.v_packed.size());
+         }
       } else if (const auto *mir = dynamic_cast<const MirrorBRDF *>(brdf)) {
          d.brdf = PTX_BRDF_MIRROR;
          d.refl[0] = mir->refl.r; d.refl[1] = mir->refl.g; d.refl[2] = mir->refl.b;
@@ -110,6 +131,8 @@ void GpuRunner::Run(int threadId) {
    std::memset(&sd, 0, sizeof(sd));
    sd.meshes = meshes.data();
    sd.num_meshes = (uint32_t)meshes.size();
+   sd.textures = textures.data();
+   sd.num_textures = (uint32_t)textures.size();
    std::vector<uint32_t> pairs;
    if (use_scene_bvh_) {   // the tree bvh::bound + bvh::compact already made for this scene
       const compact_bvh *cb = Scene->bvh_root.compact_root;

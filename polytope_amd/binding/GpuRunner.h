@@ -14,8 +14,9 @@
 //   runner->Output();
 //
 // Supported descriptions are the ones the reference's CPU path renders:
-// CenterSampler, PathTraceIntegrator, LambertBRDF / MirrorBRDF materials
-// without textures, emitters (Mesh::spd), an optional ColorSkybox and the
+// CenterSampler, PathTraceIntegrator, LambertBRDF (constant or with an RGBA
+// texture and the mesh's u, v) / MirrorBRDF materials, emitters (Mesh::spd),
+// an optional ColorSkybox and the
 // left-handed PerspectiveCamera the pbrt parser builds.  Anything else makes
 // Run() fail with status() != 0 and a message in error(); the film is then
 // left empty.  Lambert bounces draw from the library's counter-based RNG

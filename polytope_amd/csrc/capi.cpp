@@ -45,7 +45,8 @@ int upload(ptx_scene *s, const std::vector<T> &v, const T **dst) {
 
 }  // namespace
 
-static_assert(sizeof(ptx_mesh_desc) == 256, "ptx_mesh_desc layout (Python and oracle mirrors)");
+static_assert(sizeof(ptx_mesh_desc) == 280, "ptx_mesh_desc layout (Python and oracle mirrors)");
+static_assert(sizeof(ptx_texture_desc) == 16, "ptx_texture_desc layout (Python mirror)");
 
 uint32_t lds_depth_of(const ptx_device *dev) { return dev->tuning.lds_stack ? dev->tuning.lds_stack : 12; }
 
@@ -140,6 +141,12 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    if (desc->num_meshes == 0 || !desc->meshes) return set_error(PTX_ERR_INVALID, "scene has no meshes");
    if (desc->num_lights && !desc->lights) return set_error(PTX_ERR_INVALID, "num_lights > 0 with NULL lights");
    if (desc->num_lights > 64) return set_error(PTX_ERR_INVALID, "at most 64 point lights");
+   if (desc->num_textures && !desc->textures) return set_error(PTX_ERR_INVALID, "num_textures > 0 with NULL textures");
+   for (uint32_t t = 0; t < desc->num_textures; t++) {
+      const ptx_texture_desc &x = desc->textures[t];
+      if (!x.rgba || x.width < 1 || x.height < 1 || x.width > 32767 || x.height > 32767)
+         return set_error(PTX_ERR_INVALID, "texture " + std::to_string(t) + " needs 1..32767 x 1..32767 RGBA8 texels");
+   }
    for (uint32_t i = 0; i < desc->num_meshes; i++) {
       const ptx_mesh_desc &m = desc->meshes[i];
       if (!m.x || !m.y || !m.z || !m.fv0 || !m.fv1 || !m.fv2)
@@ -152,16 +159,30 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
          if (m.fv0[f] >= m.num_vertices || m.fv1[f] >= m.num_vertices || m.fv2[f] >= m.num_vertices)
             return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " face " + std::to_string(f) +
                                                   " indexes past its vertices");
+      if (m.texture > desc->num_textures)
+         return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " names a texture the scene lacks");
+      if (m.texture > 0 && !m.is_light) {
+         if (m.brdf != PTX_BRDF_LAMBERT)
+            return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + ": only Lambert meshes take a texture");
+         if (!m.u || !m.v)
+            return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " has a texture but no texture coordinates");
+         for (uint32_t f = 0; f < m.num_faces; f++)
+            if (m.fv0[f] >= m.num_uv || m.fv1[f] >= m.num_uv || m.fv2[f] >= m.num_uv)
+               return set_error(PTX_ERR_INVALID, "mesh " + std::to_string(i) + " face " + std::to_string(f) +
+                                                     " indexes past its texture coordinates");
+      }
    }
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    const double t0 = now_s();
    auto s = std::make_unique<ptx_scene>();
    s->dev = dev;
    s->num_meshes = desc->num_meshes;
-   for (uint32_t i = 0; i < desc->num_meshes; i++)
-      s->has_glossy |= desc->meshes[i].brdf == PTX_BRDF_GLOSSY && !desc->meshes[i].is_light;
    std::vector<WorldMesh> meshes(desc->num_meshes);
-   for (uint32_t i = 0; i < desc->num_meshes; i++) world_vertices(desc->meshes[i], meshes[i]);
+   for (uint32_t i = 0; i < desc->num_meshes; i++) {
+      world_vertices(desc->meshes[i], meshes[i]);
+      s->has_glossy |= meshes[i].rec.brdf == PTX_BRDF_GLOSSY && !meshes[i].rec.is_light;
+      s->has_texture |= (meshes[i].rec.flags & kRecTextured) != 0;
+   }
    double build_s = 0;
    if (desc->nodes && desc->leaf_pairs) {
       s->bvh.nodes.assign(desc->nodes, desc->nodes + desc->num_nodes);
@@ -178,6 +199,26 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    if ((rc = upload(s.get(), img.nodes, &s->d.nodes)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.tris, &s->d.tris)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), s->bvh.pairs, &s->d.pairs)) != PTX_OK) return rc;
+   if (s->has_texture) {
+      // one device copy per texture that a mesh uses, then each textured mesh's
+      // per-face corner coordinates; the mesh records point at both
+      std::vector<const uint32_t *> texels(desc->num_textures, nullptr);
+      for (uint32_t i = 0; i < desc->num_meshes; i++) {
+         const int32_t t = meshes[i].texture;
+         if (t < 0) continue;
+         const ptx_texture_desc &x = desc->textures[t];
+         if (!texels[t]) {
+            std::vector<uint32_t> px(size_t(x.width) * x.height);
+            std::memcpy(px.data(), x.rgba, px.size() * 4);
+            if ((rc = upload(s.get(), px, &texels[t])) != PTX_OK) return rc;
+         }
+         const float *uv = nullptr;
+         if ((rc = upload(s.get(), meshes[i].uv, &uv)) != PTX_OK) return rc;
+         img.meshes[i].texels = texels[t];
+         img.meshes[i].tex_uv = uv;
+         img.meshes[i].tex_wh = x.width | x.height << 16;
+      }
+   }
    if ((rc = upload(s.get(), img.meshes, &s->d.meshes)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.normals, &s->d.normals)) != PTX_OK) return rc;
    if (desc->num_lights) {
@@ -231,7 +272,7 @@ int ptx_bvh_build(const ptx_mesh_desc *meshes, uint32_t num_meshes, uint32_t thr
                   uint32_t *num_nodes, uint32_t *leaf_pairs, uint32_t *num_pairs) {
    if (!meshes || !num_meshes || !num_nodes || !num_pairs) return set_error(PTX_ERR_INVALID, "NULL argument");
    std::vector<WorldMesh> wm(num_meshes);
-   for (uint32_t i = 0; i < num_meshes; i++) world_vertices(meshes[i], wm[i]);
+   for (uint32_t i = 0; i < num_meshes; i++) world_vertices(meshes[i], wm[i], false);
    HostBvh bvh;
    build_reference_bvh(wm, threads, bvh);
    const uint32_t nn = (uint32_t)bvh.nodes.size(), np = (uint32_t)(bvh.pairs.size() / 2);

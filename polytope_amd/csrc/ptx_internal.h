@@ -2,6 +2,7 @@
 // libptx_hip.so.  Not part of the C ABI (include/ptx.h).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -35,16 +36,26 @@ static_assert(sizeof(Node64) == 64, "Node64 is 64 bytes");
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 
-// Per-mesh shading record.  Glossy meshes that are not lights keep the lobe's
-// constant reflectance in refl and its roughness in emission[0].
+// Per-mesh shading record, 64 bytes (words 0-8 are what a shading step reads
+// first).  Glossy meshes that are not lights keep the lobe's constant
+// reflectance in refl and its roughness in emission[0].  A textured Lambert
+// mesh (kRecTextured) takes its reflectance from its texture instead of refl.
+constexpr uint32_t kRecNormals = 1u, kRecTextured = 2u;
 struct MeshRec {
    int32_t brdf;
    float refl[3];
    int32_t is_light;
    float emission[3];
-   uint32_t has_normals;
-   uint32_t normal_base;   // index of the mesh's first corner normal / 3 (per face)
+   uint32_t flags;           // kRecNormals | kRecTextured
+   uint32_t tex_wh;          // textured: texture width | height << 16
+   const float *tex_uv;      // textured: (u, v) of each face's corners 0, 1, 2 (6 floats per face, face order)
+   const uint32_t *texels;   // textured: RGBA8 texels, rows top to bottom
+   uint32_t pad[2];
 };
+static_assert(sizeof(MeshRec) == 64, "MeshRec layout");
+static_assert(offsetof(MeshRec, flags) == 32 && offsetof(MeshRec, tex_wh) == 36 && offsetof(MeshRec, tex_uv) == 40 &&
+                    offsetof(MeshRec, texels) == 48,
+              "MeshRec word offsets");
 
 // Device view of one scene.  The first 48 bytes are what a traversal needs
 // (loaded per lane by the render path's extend stage as three 16-byte loads).
@@ -100,11 +111,15 @@ struct WorldMesh {
    std::vector<float> wv;                 // 9 floats per face (world v0, v1, v2)
    bool has_normals = false;
    std::vector<float> wn;                 // 9 floats per face (world corner normals)
+   std::vector<float> uv;                 // textured: 6 floats per face (u, v of corners 0, 1, 2)
+   int32_t texture = -1;                  // textured: index into the scene's texture table
    MeshRec rec{};
 };
 
 // Transform::ApplyInPlace(Point&) semantics (src/cpu/structures/Transform.cpp:69-91).
-void world_vertices(const ptx_mesh_desc &m, WorldMesh &out);
+// shading: also the mesh record and a textured mesh's corner coordinates (whose
+// indices the caller has checked against num_uv).
+void world_vertices(const ptx_mesh_desc &m, WorldMesh &out, bool shading = true);
 
 // Decision-identical re-implementation of bvh::bound + bvh::compact
 // (src/cpu/acceleration/bvh.cpp:36-395, 652-698), multithreaded.

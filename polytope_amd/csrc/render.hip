@@ -288,18 +288,57 @@ __device__ __forceinline__ Frame frame_of(const DevScene &S, uint32_t tri, const
    Ray r;
    r.dx = d[0]; r.dy = d[1]; r.dz = d[2];
    const float *cn = nullptr;
-   if (S.normals && S.meshes[mesh].has_normals) cn = S.normals + 9 * (size_t)tri;
+   if (S.normals && (S.meshes[mesh].flags & kRecNormals)) cn = S.normals + 9 * (size_t)tri;
    Frame fr;
    hit_frame(S.tris + 9 * (size_t)tri, cn, r, u, v, w, after, fr);
    return fr;
 }
 
+// ---------------------------------------------------------------- textures
+
+// (unsigned int)f where the conversion is defined (0 <= f < 2^32); 0 for a
+// negative or NaN f, 0xFFFFFFFF above.
+__device__ __forceinline__ uint32_t texel_trunc(float f) {
+   return f > 0.f ? (f < 4294967296.f ? (uint32_t)f : 0xFFFFFFFFu) : 0u;
+}
+
+// Texel of a hit on a textured Lambert mesh (LambertBRDF::sample,
+// lambert_brdf.h:49-56): the texture coordinates Mesh::intersect interpolates
+// (mesh.cpp:342-353; u, v, w weight the face's corners 0, 1, 2), then
+// texture::evaluate_rgba's flat index row * W + column (texture.h:25-31).
+// Where that index falls outside the image (the reference reads outside its
+// texel array there) the column and the row clamp to the image.
+__device__ __forceinline__ uint32_t texel_index(const float uv[6], float u, float v, float w, uint32_t wh) {
+   const float tu = (uv[0] * u + uv[2] * v) + uv[4] * w;
+   const float tv = (uv[1] * u + uv[3] * v) + uv[5] * w;
+   const uint32_t W = wh & 0xFFFFu, H = wh >> 16;
+   const uint32_t col = texel_trunc(tu * (float)W), row = texel_trunc((1.0f - tv) * (float)H);
+   const uint64_t flat = (uint64_t)row * W + col;
+   if (flat < (uint64_t)W * H) return (uint32_t)flat;
+   return min(row, H - 1u) * W + min(col, W - 1u);
+}
+
+// ReflectanceSpectrum(int r, int g, int b) (spectrum.h:17-18) of an RGBA8 texel.
+__device__ __forceinline__ void texel_refl(uint32_t t, float refl[3]) {
+   refl[0] = (float)(t & 255u) * (1.0f / 255.0f);
+   refl[1] = (float)((t >> 8) & 255u) * (1.0f / 255.0f);
+   refl[2] = (float)((t >> 16) & 255u) * (1.0f / 255.0f);
+}
+
+// The RGBA8 texel under a hit on face `face` (mesh order) of textured mesh M.
+__device__ __forceinline__ uint32_t hit_texel(const MeshRec &M, uint32_t face, float u, float v, float w) {
+   const float *c = M.tex_uv + 6 * (size_t)face;
+   const float uv[6] = {c[0], c[1], c[2], c[3], c[4], c[5]};
+   return M.texels[texel_index(uv, u, v, w, M.tex_wh)];
+}
+
 // ---------------------------------------------------------------- primary hits
 
 // Record per pixel (16 words): w0 = kind | mesh << 2 (kind 0 = the path
-// bounces, 1 = it ends here with radiance w1..w3 for src = 1); for kind 0:
-// w1..3 Location, w4..6 tangent1, w7..9 normal, w10..12 tangent2, w13..15 the
-// normalised local incoming direction.
+// bounces, 1 = it ends here with radiance w1..w3 for src = 1), or, for a
+// textured Lambert hit, 2 | RGB texel << 8; for kinds 0 and 2: w1..3 Location,
+// w4..6 tangent1, w7..9 normal, w10..12 tangent2, w13..15 the normalised local
+// incoming direction.
 struct PrimaryPolicy {
    struct Ctx {
       uint32_t dbg;
@@ -346,7 +385,10 @@ struct PrimaryPolicy {
             const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
             float lin[3];
             world_to_local(fr, d, lin);
-            rec[0] = __uint_as_float(mesh << 2);
+            uint32_t w0 = mesh << 2;
+            if (M.flags & kRecTextured)
+               w0 = 2u | (hit_texel(M, S.pairs[2 * (size_t)s.hit + 1], s.hu, s.hv, s.hw) & 0xFFFFFFu) << 8;
+            rec[0] = __uint_as_float(w0);
             rec[1] = fr.lx; rec[2] = fr.ly; rec[3] = fr.lz;
             rec[4] = fr.t1x; rec[5] = fr.t1y; rec[6] = fr.t1z;
             rec[7] = fr.nx; rec[8] = fr.ny; rec[9] = fr.nz;
@@ -434,7 +476,13 @@ __global__ void __launch_bounds__(256) k_round_start(const Chunk *chunks, uint32
             write_res(res, slot, Radiance{a.y, a.z, a.w});
          } else {
             const DevScene &S = scenes[J.scene];
-            const MeshRec &M = S.meshes[w0 >> 2];
+            MeshRec M;
+            if (w0 & 2u) {                      // textured Lambert: the texel rides in the record
+               M.brdf = PTX_BRDF_LAMBERT;
+               texel_refl(w0 >> 8, M.refl);
+            } else {
+               M = S.meshes[w0 >> 2];
+            }
             Frame fr;
             fr.lx = a.y; fr.ly = a.z; fr.lz = a.w;
             fr.t1x = b.x; fr.t1y = b.y; fr.t1z = b.z;
@@ -551,13 +599,16 @@ __global__ void __launch_bounds__(256) k_shade(const float *queue_in, const uint
             write_res(res, path, miss_radiance(S, src));
          } else {
             const uint32_t mesh = S.pairs[2 * (size_t)tri];
-            const MeshRec &M = S.meshes[mesh];
-            if (M.is_light) {
-               write_res(res, path, Radiance{(M.emission[0] + 0.f) * src[0], (M.emission[1] + 0.f) * src[1],
-                                             (M.emission[2] + 0.f) * src[2]});
+            const MeshRec &Mg = S.meshes[mesh];
+            if (Mg.is_light) {
+               write_res(res, path, Radiance{(Mg.emission[0] + 0.f) * src[0], (Mg.emission[1] + 0.f) * src[1],
+                                             (Mg.emission[2] + 0.f) * src[2]});
             } else if (b >= J.max_depth) {
                write_res(res, path, Radiance{0.f, 0.f, 0.f});
             } else {
+               MeshRec M = Mg;
+               if (M.flags & kRecTextured)
+                  texel_refl(hit_texel(M, S.pairs[2 * (size_t)tri + 1], h0.z, h0.w, h1.x), M.refl);
                const Frame fr = frame_of(S, tri, dir, h0.z, h0.w, h1.x, __float_as_uint(h1.y), mesh);
                float lin[3], wout[3];
                world_to_local(fr, dir, lin);
@@ -634,7 +685,8 @@ __global__ void k_zero_floats(float *a, size_t n) {
 // ---------------------------------------------------------------- plain path
 
 // One sample of every pixel: one thread per pixel path, all bounces inline.
-template <int kLds, bool kNee>
+// kTex: the scene has textured meshes.
+template <int kLds, bool kNee, bool kTex>
 __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sample, uint4 *spill) {
    __shared__ uint32_t lds_ref[kLds * 256];
    __shared__ float lds_near[kLds * 256];
@@ -666,13 +718,16 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
             break;
          }
          const uint32_t mesh = S.pairs[2 * (size_t)s.hit];
-         const MeshRec &M = S.meshes[mesh];
-         if (M.is_light) {
-            L = Radiance{(M.emission[0] + 0.f) * src[0], (M.emission[1] + 0.f) * src[1],
-                         (M.emission[2] + 0.f) * src[2]};
+         const MeshRec &Mg = S.meshes[mesh];
+         if (Mg.is_light) {
+            L = Radiance{(Mg.emission[0] + 0.f) * src[0], (Mg.emission[1] + 0.f) * src[1],
+                         (Mg.emission[2] + 0.f) * src[2]};
             break;
          }
          if (b >= J.max_depth) break;
+         MeshRec M = Mg;
+         if (kTex && (M.flags & kRecTextured))
+            texel_refl(hit_texel(M, S.pairs[2 * (size_t)s.hit + 1], s.hu, s.hv, s.hw), M.refl);
          const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
          if (nee && M.brdf == PTX_BRDF_LAMBERT) {
             for (uint32_t j = 0; j < S.num_lights; j++) {
@@ -725,7 +780,10 @@ enum : uint32_t {
    kPIdle = 0, kPPrim = 1, kPTrace = 2, kPShade1 = 3, kPShade2 = 4, kPShade3 = 5,
    kPLight = 6,    // next-event estimation: add the last light's term, set up the next shadow ray
    kPShadow = 7,   // next-event estimation: shadow-ray traversal
-   kPHold = 8      // this iteration only: a lane that waits for its wave to gather
+   kPHold = 8,     // this iteration only: a lane that waits for its wave to gather
+   // a hit on a textured mesh: its texture record, its corner coordinates, its
+   // texel, then the faces (Shade3 with normals, Shade4 without)
+   kPTex1 = 9, kPTex2 = 10, kPTex3 = 11, kPShade4 = 12
 };
 // Lane word: bits 0-3 state, bit 4 the reference's debug pixel, bits 5-10 the
 // path's chunk in the round, bits 11-30 its bounce index, bit 31 set when the
@@ -782,7 +840,8 @@ struct NeeRecord {
    float4 a, b, c;   // a = P.xyz, wout.x; b = wout.yz, k.xy; c = k.z, n.xyz
 };
 
-template <int kLds, bool kNee, bool kGlossy>
+// kMat: some scene of the round has glossy or textured meshes.
+template <int kLds, bool kNee, bool kMat>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
         uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
@@ -817,8 +876,10 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
    s.sp = 0;
    uint32_t mode = kPIdle;       // lane word
    uint32_t slot = 0;
-   uint32_t t0 = 0;              // Shade1 -> Shade2: mesh; Shade2 -> Shade3: brdf; Light/Shadow: next light
-   float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2 -> Shade3: reflectance; Light/Shadow: visible, last vertex
+   uint32_t t0 = 0;              // Shade1 -> Shade2 -> Tex1: mesh; Shade2/Tex3 -> Shade3/4: brdf; Light/Shadow: next light
+   float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2/Tex3 -> Shade3/4: reflectance; Light/Shadow: visible, last vertex
+   // Shade1 .. Tex3 also keep the hit's face, texture record and texel address in
+   // the ray origin and reciprocal-direction words, which a found hit no longer needs
    float src[3] = {1.f, 1.f, 1.f};
    float direct[3] = {0.f, 0.f, 0.f};    // next-event sum of the path (kNee)
    float pend[3] = {0.f, 0.f, 0.f};      // the light term a running shadow ray decides (kNee)
@@ -859,15 +920,16 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
       }
       const uint32_t m0 = mode & kPModeMask;
       if (!__any(m0 != kPIdle)) break;
-      const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3;
+      const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3 || (kMat && m0 == kPShade4);
       const bool tracing = m0 == kPTrace || m0 == kPShadow;
-      const bool run_heavy = (uint32_t)__popcll(__ballot(heavy)) >= shade_min ||
-                             !__any(tracing || m0 == kPShade1 || m0 == kPLight);
+      // single-load steps that never wait for the wave
+      const bool light_step = m0 == kPShade1 || m0 == kPLight || (kMat && m0 >= kPTex1 && m0 <= kPTex3);
+      const bool run_heavy = (uint32_t)__popcll(__ballot(heavy)) >= shade_min || !__any(tracing || light_step);
       // leaf steps gather the same way; lanes at interior nodes, in Shade1 or in
       // Light always step, so some lane progresses whenever any lane holds
       const bool at_leaf = tracing && (s.cur & kLeafBit) != 0;
       const bool run_leaf = (uint32_t)__popcll(__ballot(at_leaf)) >= leaf_min ||
-                            !__any((tracing && !at_leaf) || m0 == kPShade1 || m0 == kPLight);
+                            !__any((tracing && !at_leaf) || light_step);
       const uint32_t m = (heavy && !run_heavy) || (at_leaf && !run_leaf) ? kPHold : m0;
       // ------------------------------------------------------------ fetch
       Fetched f;
@@ -885,7 +947,14 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          f.q[8] = q2.x; f.q[9] = q2.y; f.q[10] = q2.z; f.q[11] = q2.w;
          f.q[12] = q3.x; f.q[13] = q3.y; f.q[14] = q3.z; f.q[15] = q3.w;
       } else if (m == kPShade1) {
-         f.q[0] = __uint_as_float(as_global(ssc[sch[lw_chunk(mode)].scene].pairs)[2 * (size_t)s.hit]);
+         const PTX_GLOBAL uint32_t *pp = as_global(ssc[sch[lw_chunk(mode)].scene].pairs) + 2 * (size_t)s.hit;
+         if (kMat) {                                // (mesh, face)
+            const u32x2 pr = *reinterpret_cast<const PTX_GLOBAL u32x2 *>(pp);
+            f.q[0] = __uint_as_float(pr.x);
+            f.q[1] = __uint_as_float(pr.y);
+         } else {
+            f.q[0] = __uint_as_float(pp[0]);
+         }
       } else if (m == kPShade2) {
          const PTX_GLOBAL uint32_t *mr =
                as_global(reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0));
@@ -899,6 +968,24 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          const PTX_GLOBAL float *nrm = as_global(ssc[sch[lw_chunk(mode)].scene].normals + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) { f.q[k] = v[k]; f.q[9 + k] = nrm[k]; }
+      } else if (kMat && m == kPShade4) {
+         const PTX_GLOBAL float *v = as_global(tris + 9 * (size_t)s.hit);
+#pragma unroll
+         for (int k = 0; k < 9; k++) f.q[k] = v[k];
+      } else if (kMat && m == kPTex1) {            // MeshRec words 9-13: tex_wh, tex_uv, texels
+         const PTX_GLOBAL uint32_t *mr =
+               as_global(reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0));
+#pragma unroll
+         for (int k = 0; k < 5; k++) f.q[k] = __uint_as_float(mr[9 + k]);
+      } else if (kMat && m == kPTex2) {            // the face's corner coordinates
+         const PTX_GLOBAL float *uv = as_global(reinterpret_cast<const float *>(
+               (uint64_t)__float_as_uint(s.r.ix) | ((uint64_t)__float_as_uint(s.r.iy) << 32)));
+#pragma unroll
+         for (int k = 0; k < 6; k++) f.q[k] = uv[k];
+      } else if (kMat && m == kPTex3) {            // the texel
+         const PTX_GLOBAL uint32_t *tx = as_global(reinterpret_cast<const uint32_t *>(
+               (uint64_t)__float_as_uint(s.r.ix) | ((uint64_t)__float_as_uint(s.r.iy) << 32)));
+         f.q[0] = __uint_as_float(*tx);
       } else if (kNee && m == kPLight) {
          const PTX_GLOBAL f32x4u *rec = as_global(reinterpret_cast<const f32x4u *>(nee_scratch + 3 * (size_t)gtid));
          const f32x4u a = rec[0], b = rec[1], c = rec[2];
@@ -976,13 +1063,18 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             mode = kPIdle;
          } else {
             const DevScene &S = ssc[c.scene];
-            const PTX_GLOBAL uint32_t *mw = as_global(reinterpret_cast<const uint32_t *>(S.meshes + (w0 >> 2)));
             MeshRec M;
-            M.brdf = (int32_t)mw[0];
-            M.refl[0] = __uint_as_float(mw[1]);
-            M.refl[1] = __uint_as_float(mw[2]);
-            M.refl[2] = __uint_as_float(mw[3]);
-            if (kGlossy) M.emission[0] = __uint_as_float(mw[5]);       // glossy roughness
+            if (kMat && (w0 & 2u)) {               // textured Lambert: the texel rides in the record
+               M.brdf = PTX_BRDF_LAMBERT;
+               texel_refl(w0 >> 8, M.refl);
+            } else {
+               const PTX_GLOBAL uint32_t *mw = as_global(reinterpret_cast<const uint32_t *>(S.meshes + (w0 >> 2)));
+               M.brdf = (int32_t)mw[0];
+               M.refl[0] = __uint_as_float(mw[1]);
+               M.refl[1] = __uint_as_float(mw[2]);
+               M.refl[2] = __uint_as_float(mw[3]);
+               if (kMat) M.emission[0] = __uint_as_float(mw[5]);       // glossy roughness
+            }
             Frame fr;
             fr.lx = f.q[1]; fr.ly = f.q[2]; fr.lz = f.q[3];
             fr.t1x = f.q[4]; fr.t1y = f.q[5]; fr.t1z = f.q[6];
@@ -996,7 +1088,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const float kd[3] = {M.refl[0], M.refl[1], M.refl[2]};
             const float s_before[3] = {src[0], src[1], src[2]};
             float wout[3];
-            bounce<kGlossy>(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
+            bounce<kMat>(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
             nodes = S.nodes;
             tris = S.tris;
             if (nee) {
@@ -1016,16 +1108,41 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          }
       } else if (m == kPShade1) {
          t0 = __float_as_uint(f.q[0]);
+         if (kMat) s.r.ox = f.q[1];                // the hit's face, for a textured mesh
          mode = lw_with_state(mode, kPShade2);
-      } else if (m == kPShade2 || m == kPShade3) {
+      } else if (kMat && m == kPTex1) {
+         const uint32_t face = __float_as_uint(s.r.ox);
+         const uint64_t uv = ((uint64_t)__float_as_uint(f.q[1]) | ((uint64_t)__float_as_uint(f.q[2]) << 32)) +
+                             (uint64_t)face * 24u;
+         s.r.oz = f.q[0];                          // width | height << 16
+         s.r.ix = __uint_as_float((uint32_t)uv);
+         s.r.iy = __uint_as_float((uint32_t)(uv >> 32));
+         s.r.iz = f.q[3];                          // texels
+         s.r.Sx = f.q[4];
+         mode = lw_with_state(mode, kPTex2);
+      } else if (kMat && m == kPTex2) {
+         const uint32_t k = texel_index(&f.q[0], s.hu, s.hv, s.hw, __float_as_uint(s.r.oz));
+         const uint64_t tx = ((uint64_t)__float_as_uint(s.r.iz) | ((uint64_t)__float_as_uint(s.r.Sx) << 32)) +
+                             (uint64_t)k * 4u;
+         s.r.ix = __uint_as_float((uint32_t)tx);
+         s.r.iy = __uint_as_float((uint32_t)(tx >> 32));
+         mode = lw_with_state(mode, kPTex3);
+      } else if (kMat && m == kPTex3) {
+         float refl[3];
+         texel_refl(__float_as_uint(f.q[0]), refl);
+         t0 = (uint32_t)PTX_BRDF_LAMBERT;
+         t1 = refl[0]; t2 = refl[1]; t3 = refl[2];
+         const bool normals = (__float_as_uint(s.r.oy) & kRecNormals) && ssc[sch[lw_chunk(mode)].scene].normals;
+         mode = lw_with_state(mode, normals ? kPShade3 : kPShade4);
+      } else if (m == kPShade2 || m == kPShade3 || (kMat && m == kPShade4)) {
          const LdsChunk &c = sch[lw_chunk(mode)];
          const uint32_t b = lw_bounce(mode);
          bool go = true;
          if (m == kPShade2) {
             const int32_t brdf = __float_as_int(f.q[0]);
             const int32_t is_light = __float_as_int(f.q[4]);
-            const uint32_t has_normals = __float_as_uint(f.q[8]);
-            if (kGlossy) s.cnt = __float_as_uint(f.q[5]);          // glossy roughness, kept until the bounce
+            const uint32_t mflags = __float_as_uint(f.q[8]);
+            if (kMat) s.cnt = __float_as_uint(f.q[5]);             // glossy roughness, kept until the bounce
             if (is_light) {
                write_res(res, slot, fin<kNee>(Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
                                                        (f.q[7] + 0.f) * src[2]}, direct, mode));
@@ -1035,7 +1152,11 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
                write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
                mode = kPIdle;
                go = false;
-            } else if (has_normals && ssc[c.scene].normals) {
+            } else if (kMat && (mflags & kRecTextured)) {     // t0 still holds the mesh
+               s.r.oy = f.q[8];
+               mode = lw_with_state(mode, kPTex1);
+               go = false;
+            } else if ((mflags & kRecNormals) && ssc[c.scene].normals) {
                t0 = (uint32_t)brdf;
                t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
                mode = lw_with_state(mode, kPShade3);
@@ -1051,7 +1172,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             MeshRec M;
             M.brdf = (int32_t)t0;
             M.refl[0] = t1; M.refl[1] = t2; M.refl[2] = t3;
-            if (kGlossy) M.emission[0] = __uint_as_float(s.cnt);
+            if (kMat) M.emission[0] = __uint_as_float(s.cnt);
             Frame fr;
             hit_frame_sel(&f.q[0], &f.q[9], m == kPShade3, s.r, s.hu, s.hv, s.hw, s.after, fr);
             const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
@@ -1062,7 +1183,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const float s_before[3] = {src[0], src[1], src[2]};
             float lin[3], wout[3];
             world_to_local(fr, d, lin);
-            bounce<kGlossy>(M, fr, lin, pixel, sample, b, c.seed, wout, src);
+            bounce<kMat>(M, fr, lin, pixel, sample, b, c.seed, wout, src);
             if (nee) {
                const float keep[3] = {src[0], src[1], src[2]};
                src[0] = s_before[0]; src[1] = s_before[1]; src[2] = s_before[2];
@@ -1096,8 +1217,8 @@ int render_blocks_per_cu(int which) {
    hipError_t e = hipSuccess;
    if (which == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_extend<kRenderLds>, 256, 0);
    else if (which == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_primary<kRenderLds>, 256, 0);
-   else if (which == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, false>, 256, 0);
-   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, true>, 256, 0);
+   else if (which == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, false, false>, 256, 0);
+   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, true, false>, 256, 0);
    return e == hipSuccess && b > 0 ? b : 1;
 }
 
@@ -1163,23 +1284,32 @@ uint32_t paths_max_scenes() { return dev::kMaxRoundScenes; }
 
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-                  uint32_t leaf_min, float4 *nee_scratch, bool glossy, uint32_t grid, hipStream_t st) {
+                  uint32_t leaf_min, float4 *nee_scratch, bool materials, uint32_t grid, hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
 #define PTX_LAUNCH_PATHS(N, G)                                                                                     \
    hipLaunchKernelGGL((dev::k_paths<kRenderLds, N, G>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, \
                       scenes, nscenes, res, work, spill, max_stack, shade_min, leaf_min, nee_scratch)
-   if (nee_scratch && glossy) PTX_LAUNCH_PATHS(true, true);
+   if (nee_scratch && materials) PTX_LAUNCH_PATHS(true, true);
    else if (nee_scratch) PTX_LAUNCH_PATHS(true, false);
-   else if (glossy) PTX_LAUNCH_PATHS(false, true);
+   else if (materials) PTX_LAUNCH_PATHS(false, true);
    else PTX_LAUNCH_PATHS(false, false);
 #undef PTX_LAUNCH_PATHS
 }
 
-void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {
+namespace {
+template <bool T>
+void launch_mega_t(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {
    if (J.flags & PTX_JOB_NEE)
-      hipLaunchKernelGGL((dev::k_mega<kRenderLds, true>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+      hipLaunchKernelGGL((dev::k_mega<kRenderLds, true, T>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
    else
-      hipLaunchKernelGGL((dev::k_mega<kRenderLds, false>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+      hipLaunchKernelGGL((dev::k_mega<kRenderLds, false, T>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+}
+}  // namespace
+
+void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, bool textured, uint32_t grid,
+                 hipStream_t st) {
+   if (textured) launch_mega_t<true>(S, J, sample, spill, grid, st);
+   else launch_mega_t<false>(S, J, sample, spill, grid, st);
 }
 
 }  // namespace ptx

@@ -29,7 +29,7 @@ inline float sop(float a, float b, float c, float d) {      // src/cpu/constants
 }
 }  // namespace
 
-void world_vertices(const ptx_mesh_desc &m, WorldMesh &out) {
+void world_vertices(const ptx_mesh_desc &m, WorldMesh &out, bool shading) {
    out.nf = m.num_faces;
    out.wv.resize(size_t(9) * m.num_faces);
    const float *M = m.object_to_world;
@@ -82,7 +82,23 @@ void world_vertices(const ptx_mesh_desc &m, WorldMesh &out) {
       for (int c = 0; c < 3; c++) out.rec.refl[c] = m.specular[c] * (1 - r) + m.refl[c] * r;
       out.rec.emission[0] = r;
    }
-   out.rec.has_normals = out.has_normals ? 1u : 0u;
+   out.rec.flags = out.has_normals ? kRecNormals : 0u;
+   out.texture = -1;
+   out.uv.clear();
+   if (shading && m.texture > 0 && m.brdf == PTX_BRDF_LAMBERT && !m.is_light && m.u && m.v) {
+      // LambertBRDF(texture) (lambert_brdf.h:31-56): the face's corner texture
+      // coordinates u_packed / v_packed[fv0, fv1, fv2] (mesh.cpp:342-351)
+      out.texture = (int32_t)m.texture - 1;
+      out.rec.flags |= kRecTextured;
+      out.uv.resize(size_t(6) * m.num_faces);
+      for (size_t f = 0; f < m.num_faces; f++) {
+         const uint32_t ix[3] = {m.fv0[f], m.fv1[f], m.fv2[f]};
+         for (int c = 0; c < 3; c++) {
+            out.uv[6 * f + 2 * c] = m.u[ix[c]];
+            out.uv[6 * f + 2 * c + 1] = m.v[ix[c]];
+         }
+      }
+   }
 }
 
 int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostImage &img) {

@@ -73,4 +73,5 @@ struct ptx_scene {
    ptx_scene_info info{};
    uint32_t num_meshes = 0;
    bool has_glossy = false;          // a mesh that is not a light has PTX_BRDF_GLOSSY
+   bool has_texture = false;         // a Lambert mesh that is not a light has a texture
 };

@@ -34,7 +34,7 @@ __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, 
       if (s.hit != 0xFFFFFFFFu) {
          Frame f;
          const float *cn = sc.normals ? sc.normals + 9 * (size_t)s.hit : nullptr;
-         if (cn && !sc.meshes[h.mesh].has_normals) cn = nullptr;
+         if (cn && !(sc.meshes[h.mesh].flags & kRecNormals)) cn = nullptr;
          hit_frame(sc.tris + 9 * (size_t)s.hit, cn, s.r, s.hu, s.hv, s.hw, s.after, f);
          fr = ptx_hit_frame{{f.lx, f.ly, f.lz}, {f.nx, f.ny, f.nz}, {f.t1x, f.t1y, f.t1z},
                             {f.t2x, f.t2y, f.t2z}, {f.ex, f.ey, f.ez}};

@@ -19,6 +19,7 @@ namespace ptx {
 namespace dev {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 template <int kLds>
 struct Stack {

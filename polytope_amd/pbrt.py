@@ -43,8 +43,18 @@ The reference's quirks are kept, because they decide which scene it renders:
   next-event estimation; they do not change anything else.
 
 The "plastic" material is the reference's GlossyBRDF (Kd, Ks, roughness).
-Not supported, with an error: textures, shapes without a material, and
-anything the reference itself would stop on.
+``Texture "name" "color" "imagemap"`` directives load PNG files as the
+reference does (pbrt_parser.cpp:730-776, decoded to RGBA8 as lodepng does,
+see png.py) and ``Material "matte" "texture Kd"`` makes a textured Lambert
+material (pbrt_parser.cpp:866-875); its meshes take their texture coordinates
+from their PLY files' u/v (or s/t) vertex properties.  The reference marks
+every PLY mesh as having texture coordinates, uninitialised when the file has
+none (mesh_parsers.cpp:320, 340); a textured mesh without them, or one whose
+texture is a "float" (one byte per texel, which the reference indexes four
+bytes at a time), cannot be shaded.
+
+Not supported, with an error: shapes without a material, and anything the
+reference itself would stop on.
 """
 from __future__ import annotations
 
@@ -59,7 +69,8 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from .scenes import GLOSSY, LAMBERT, MIRROR, Camera, MeshData
+from .png import PngError, read_png
+from .scenes import GLOSSY, LAMBERT, MIRROR, Camera, MeshData, Texture
 
 f32 = np.float32
 
@@ -462,6 +473,13 @@ class Geometry:
     nz: List[np.ndarray] = field(default_factory=list)
     faces: List[np.ndarray] = field(default_factory=list)
     has_vertex_normals: bool = False
+    # u_packed / v_packed: one entry per vertex a PLY file declares (not per
+    # stored vertex: Q6 stores binary vertices twice but their u, v once), and
+    # whether the file gave them
+    u: List[np.ndarray] = field(default_factory=list)
+    v: List[np.ndarray] = field(default_factory=list)
+    uv_given: List[bool] = field(default_factory=list)
+    has_vertex_uvs: bool = False
 
     def add_vertices(self, x, y, z, nx=None, ny=None, nz=None) -> None:
         self.x.append(np.asarray(x, np.float32))
@@ -474,6 +492,17 @@ class Geometry:
 
     def add_faces(self, f: np.ndarray) -> None:
         self.faces.append(np.asarray(f, np.uint32).reshape(-1, 3))
+
+    def add_uv(self, u, v, given: bool) -> None:
+        self.u.append(np.asarray(u, np.float32))
+        self.v.append(np.asarray(v, np.float32))
+        self.uv_given.append(given)
+
+    def uv(self) -> Optional[np.ndarray]:
+        """(N, 2) texture coordinates when every vertex got them from its file."""
+        if not self.has_vertex_uvs or not self.uv_given or not all(self.uv_given):
+            return None
+        return np.stack([np.concatenate(self.u), np.concatenate(self.v)], 1).astype(np.float32)
 
     def arrays(self):
         cat = lambda parts: np.concatenate(parts) if parts else np.zeros(0, np.float32)
@@ -577,7 +606,9 @@ def read_ply(path: str, geom: Geometry) -> None:
     if sum(a in seen for a in ("u", "v")) not in (0, 2):
         raise PbrtError(f"{path}: Header has one of u, v")
     has_normals = nn == 3
+    has_uv = "u" in seen and "v" in seen
     geom.has_vertex_normals = has_normals
+    geom.has_vertex_uvs = True        # set for every PLY file (mesh_parsers.cpp:320, 340)
     endian = ">" if mode == "binary_big_endian" else "<"
     for el in elements:
         if el["kind"] == "vertex":
@@ -601,6 +632,10 @@ def read_ply(path: str, geom: Geometry) -> None:
                 pos += size
             col = {p: vals[:, k] for k, p in enumerate(props) if p != "unknown"}
             x, y, z = col["x"], col["y"], col["z"]
+            if has_uv:
+                geom.add_uv(col["u"], col["v"], True)
+            else:
+                geom.add_uv(np.zeros(n, np.float32), np.zeros(n, np.float32), False)
             if has_normals:
                 geom.add_vertices(x, y, z, col["nx"], col["ny"], col["nz"])
             elif mode == "ascii":
@@ -699,6 +734,13 @@ class _Material:
     why: str = ""
     specular: Tuple[float, float, float] = (0.25, 0.25, 0.25)
     roughness: float = 0.1
+    texture: Optional[Texture] = None
+
+
+@dataclass(eq=False)
+class _Texture:
+    texture: Texture
+    grey: bool                    # a "float" texture: one byte per texel
 
 
 @dataclass(eq=False)
@@ -727,8 +769,9 @@ class SceneFile:
     max_depth: int
     film_filename: str
     point_lights: List[PointLight]
-    # meshes the GPU path can trace but not shade, with the reason (no material,
-    # a textured or glossy one); they carry the default Lambert 0.5 material
+    # meshes the GPU path can trace but not shade, with the reason (no
+    # material, a texture without texture coordinates, ...); they carry the
+    # default Lambert 0.5 material
     unrenderable: List[Tuple[int, str]] = field(default_factory=list)
 
     @property
@@ -851,6 +894,7 @@ def _parse(directives: List[Directive], base: str, fname: str, strict: bool) -> 
     sky = None
     meshes: List[_Mesh] = []
     points: List[PointLight] = []
+    textures: Dict[str, _Texture] = {}
 
     def add_mesh(o2w: Xf, w2o: np.ndarray, geom: Geometry) -> None:
         meshes.append(_Mesh(o2w, w2o, geom, None if light is not None else material, light))
@@ -903,7 +947,7 @@ def _parse(directives: List[Directive], base: str, fname: str, strict: bool) -> 
             else:
                 named.append(_Material(LAMBERT, (0.5, 0.5, 0.5), d.type))
         elif ident == "Material":
-            material = _material(d)
+            material = _material(d, textures)
         elif ident == "NamedMaterial":
             for m in named:
                 if m.name == d.type:
@@ -945,7 +989,9 @@ def _parse(directives: List[Directive], base: str, fname: str, strict: bool) -> 
             if xf_stack:
                 xf = xf_stack.pop()
         elif ident == "Texture":
-            pass                     # only a material that uses a texture is refused (when a shape uses it)
+            if d.name in textures:
+                raise PbrtError(f"Texture name [{d.name}] has already previously been defined.")
+            textures[d.name] = _texture(d, base)
         elif ident == "Shape":
             if d.type not in ("plymesh", "objmesh", "sphere"):
                 continue
@@ -992,12 +1038,18 @@ def _parse(directives: List[Directive], base: str, fname: str, strict: bool) -> 
             unrenderable.append((i, "no material (the reference would dereference a null material)"))
         elif m.material.brdf < 0:
             unrenderable.append((i, m.material.why))
+        elif m.material.texture is not None and m.geom.uv() is None:
+            unrenderable.append((i, "textured material on a mesh without texture coordinates in its file "
+                                    "(the reference reads uninitialised ones)"))
         else:
             md.brdf = m.material.brdf
             md.refl = m.material.refl
             if md.brdf == GLOSSY:
                 md.specular = m.material.specular
                 md.roughness = m.material.roughness
+            if m.material.texture is not None:
+                md.texture = m.material.texture
+                md.uv = m.geom.uv()
         out.append(md)
     if strict and unrenderable:
         i, why = unrenderable[0]
@@ -1006,10 +1058,50 @@ def _parse(directives: List[Directive], base: str, fname: str, strict: bool) -> 
     return SceneFile(out, cam, sky, int(spp), int(max_depth), film_filename, points, unrenderable)
 
 
-def _material(d: Directive) -> Optional[_Material]:
-    """material_directive (pbrt_parser.cpp:778-905): matte (rgb Kd) and mirror
-    (rgb Kr).  Unknown types make no material (None); materials the GPU path
-    cannot render are kept with brdf -1 and refused when a shape uses them."""
+_TEXTURE_CACHE: Dict[Tuple[str, int, int], np.ndarray] = {}
+
+
+def _load_png(path: str) -> np.ndarray:
+    """read_png with a small cache keyed by the file's identity."""
+    st = os.stat(path)
+    key = (os.path.realpath(path), st.st_mtime_ns, st.st_size)
+    img = _TEXTURE_CACHE.get(key)
+    if img is None:
+        img = read_png(path)
+        if len(_TEXTURE_CACHE) >= 8:
+            _TEXTURE_CACHE.pop(next(iter(_TEXTURE_CACHE)))
+        _TEXTURE_CACHE[key] = img
+    return img
+
+
+def _texture(d: Directive, base: str) -> _Texture:
+    """texture_directive (pbrt_parser.cpp:730-776): an imagemap PNG, decoded to
+    RGBA for "color"/"spectrum" textures and to one grey byte per texel for
+    "float" ones.  The file is looked up at base + '/' + filename."""
+    if d.type in ("color", "spectrum"):
+        grey = False
+    elif d.type == "float":
+        grey = True
+    else:
+        raise PbrtError(f"Unknown/unimplemented texture type [{d.type}]")
+    if d.class_name != "imagemap":
+        raise PbrtError(f"Unknown/unimplemented texture class [{d.class_name}]")
+    for a in d.args:
+        if a.type == "string" and a.name == "filename":
+            path = base + "/" + (a.string or "")
+            try:
+                rgba = _load_png(path)
+            except (OSError, PngError) as e:
+                raise PbrtError(f"Failed to read/decode png file [{a.string}]: {e}") from e
+            return _Texture(Texture(rgba, d.name), grey)
+    raise PbrtError("Texture directives / arguments insufficient")
+
+
+def _material(d: Directive, textures: Dict[str, _Texture]) -> Optional[_Material]:
+    """material_directive (pbrt_parser.cpp:778-905): matte (rgb Kd or texture
+    Kd), mirror (rgb Kr) and plastic.  Unknown types make no material (None);
+    materials the GPU path cannot render are kept with brdf -1 and refused
+    when a shape uses them."""
     if d.type == "matte":
         for a in d.args:
             if a.name != "Kd":
@@ -1017,7 +1109,13 @@ def _material(d: Directive) -> Optional[_Material]:
             if a.type == "rgb":
                 return _Material(LAMBERT, tuple(float(x) for x in a.floats[:3]))
             if a.type == "texture":
-                return _Material(-1, (0.0, 0.0, 0.0), why="textured Kd is not supported by the GPU path")
+                t = textures.get(a.string or "")
+                if t is None:
+                    raise PbrtError(f"Material references a texture [{a.string}] that hasn't been defined yet")
+                if t.grey:
+                    return _Material(-1, (1.0, 1.0, 1.0), why=f"texture [{a.string}] is a float texture (one byte "
+                                     "per texel, which the reference reads as four)")
+                return _Material(LAMBERT, (1.0, 1.0, 1.0), texture=t.texture)
         return _Material(-1, (0.0, 0.0, 0.0), why="Material matte without Kd makes no material in the reference")
     if d.type == "mirror":
         for a in d.args:

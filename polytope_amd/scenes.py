@@ -8,7 +8,8 @@ fixed-term Taylor evaluation in float64, not from libm.
 Scenes are lists of :class:`MeshData`, the host-side analogue of the
 reference's ``mesh_geometry`` + ``Mesh`` pair (src/cpu/shapes/mesh.h:17-113):
 packed vertices, three face-index arrays, a 4x4 object-to-world matrix and a
-material (Lambert ``refl`` or perfect mirror), optionally an emitter.
+material (Lambert ``refl`` or an image texture with per-vertex texture
+coordinates, perfect mirror, or the glossy lobe), optionally an emitter.
 """
 from __future__ import annotations
 
@@ -26,6 +27,27 @@ GLOSSY = 3        # the reference's "plastic" material (GlossyBRDF); refl holds 
 IDENTITY = np.eye(4, dtype=np.float32)
 
 
+class Texture:
+    """An RGBA8 image texture (poly::texture, src/cpu/shading/texture.h:12-42):
+    ``rgba`` is (H, W, 4) uint8, rows top to bottom.  Meshes that share one
+    Texture object share one device copy."""
+
+    def __init__(self, rgba: np.ndarray, name: str = ""):
+        a = np.ascontiguousarray(rgba, np.uint8)
+        if a.ndim != 3 or a.shape[2] != 4 or a.shape[0] < 1 or a.shape[1] < 1:
+            raise ValueError("texture must be (H, W, 4) uint8")
+        self.rgba = a
+        self.name = name
+
+    @property
+    def width(self) -> int:
+        return int(self.rgba.shape[1])
+
+    @property
+    def height(self) -> int:
+        return int(self.rgba.shape[0])
+
+
 @dataclass
 class MeshData:
     verts: np.ndarray                      # (V, 3) float32, packed vertices
@@ -39,6 +61,8 @@ class MeshData:
     specular: Tuple[float, float, float] = (0.25, 0.25, 0.25)   # glossy: Ks
     roughness: float = 0.1                                        # glossy roughness
     w2o: Optional[np.ndarray] = None       # world-to-object if given (the inverse of object_to_world otherwise)
+    texture: Optional[Texture] = None      # Lambert: the reflectance comes from this texture instead of refl
+    uv: Optional[np.ndarray] = None        # (N, 2) float32 per-vertex texture coordinates, indexed like verts
 
     @property
     def num_faces(self) -> int:
@@ -420,7 +444,10 @@ def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov:
     subset of the format the reference's parser reads (pbrt_parser.cpp,
     mesh_parsers.cpp): LookAt, Camera "perspective", Film, Sampler, Integrator
     "path", LightSource "infinite", per mesh AttributeBegin / Translate / Scale /
-    Material "matte" | "mirror" / AreaLightSource "diffuse" / Shape "plymesh".
+    Material "matte" | "mirror" | "plastic" / AreaLightSource "diffuse" /
+    Shape "plymesh".  A textured mesh's texture is written as an RGBA PNG with a
+    Texture "color" "imagemap" directive (once per texture) and its PLY carries
+    u, v vertex properties; the mesh must have per-vertex ``uv``.
     Meshes must have a translate-and-scale object-to-world (no rotation), which
     the reference composes exactly.  Floats are written with 9 significant
     digits (exact for float32).  Emission and skybox are written as value / 255
@@ -445,6 +472,15 @@ def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov:
         lines.append('LightSource "infinite" "rgb L" [ %s ]' % " ".join(_fmt(v) for v in l))
         sky_out = tuple(float(f(v) * f(255)) for v in l)
     out_meshes = []
+    tex_names = {}
+    from .png import encode_png
+    for i, m in enumerate(meshes):
+        if m.texture is not None and id(m.texture) not in tex_names:
+            tname = f"{name}_tex{len(tex_names)}"
+            with open(os.path.join(directory, tname + ".png"), "wb") as fh:
+                fh.write(encode_png(m.texture.rgba, filters=(0, 1, 2, 3, 4)))
+            tex_names[id(m.texture)] = tname
+            lines.append('Texture "%s" "color" "imagemap" "string filename" [ "%s.png" ]' % (tname, tname))
     for i, m in enumerate(meshes):
         o2w = np.asarray(m.object_to_world, np.float32)
         lin = o2w[:3, :3]
@@ -453,11 +489,20 @@ def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov:
         ply = f"{name}_mesh{i}.ply"
         v = np.asarray(m.verts, np.float32)
         fc = np.asarray(m.faces, np.uint32)
+        uv = None
+        if m.texture is not None:
+            if m.uv is None or len(m.uv) != len(v):
+                raise ValueError(f"mesh {i}: a textured mesh needs one (u, v) per vertex")
+            uv = np.asarray(m.uv, np.float32).reshape(-1, 2)
         with open(os.path.join(directory, ply), "w") as fh:
             fh.write("ply\nformat ascii 1.0\nelement vertex %d\nproperty float x\nproperty float y\n"
-                     "property float z\nelement face %d\nproperty list uchar int vertex_indices\nend_header\n"
-                     % (len(v), len(fc)))
-            fh.write("".join("%s %s %s\n" % (_fmt(a), _fmt(b), _fmt(c)) for a, b, c in v))
+                     "property float z\n%selement face %d\nproperty list uchar int vertex_indices\nend_header\n"
+                     % (len(v), "property float u\nproperty float v\n" if uv is not None else "", len(fc)))
+            if uv is None:
+                fh.write("".join("%s %s %s\n" % (_fmt(a), _fmt(b), _fmt(c)) for a, b, c in v))
+            else:
+                fh.write("".join("%s %s %s %s %s\n" % (_fmt(a), _fmt(b), _fmt(c), _fmt(tu), _fmt(tv))
+                                 for (a, b, c), (tu, tv) in zip(v, uv)))
             fh.write("".join("3 %d %d %d\n" % (a, b, c) for a, b, c in fc))
         lines.append("AttributeBegin")
         t = o2w[:3, 3]
@@ -472,6 +517,8 @@ def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov:
             lines.append('  Material "plastic" "rgb Kd" [ %s ] "rgb Ks" [ %s ] "float roughness" [ %s ]'
                          % (" ".join(_fmt(x) for x in m.refl), " ".join(_fmt(x) for x in m.specular),
                             _fmt(m.roughness)))
+        elif m.texture is not None:
+            lines.append('  Material "matte" "texture Kd" [ "%s" ]' % tex_names[id(m.texture)])
         else:
             lines.append('  Material "matte" "rgb Kd" [ %s ]' % " ".join(_fmt(x) for x in m.refl))
         emission = m.emission
@@ -481,9 +528,12 @@ def write_pbrt(directory: str, meshes: Sequence[MeshData], eye, target, up, fov:
             emission = tuple(float(f(x) * f(255)) for x in l)
         lines.append('  Shape "plymesh" "string filename" [ "%s" ]' % ply)
         lines.append("AttributeEnd")
+        textured = m.texture is not None and not m.is_light and m.brdf == LAMBERT
         out_meshes.append(MeshData(v, fc, object_to_world=o2w, brdf=m.brdf,
-                                   refl=tuple(float(f(x)) for x in m.refl), is_light=m.is_light, emission=emission,
-                                   specular=tuple(float(f(x)) for x in m.specular), roughness=float(f(m.roughness))))
+                                   refl=(1.0, 1.0, 1.0) if textured else tuple(float(f(x)) for x in m.refl),
+                                   is_light=m.is_light, emission=emission,
+                                   specular=tuple(float(f(x)) for x in m.specular), roughness=float(f(m.roughness)),
+                                   texture=m.texture if textured else None, uv=uv if textured else None))
     lines.append("WorldEnd")
     path = os.path.join(directory, name + ".pbrt")
     with open(path, "w") as fh:
@@ -542,6 +592,56 @@ def plain_render_scene(seed: int = 1, width: int = 640, height: int = 480):
         o2w = translate((xs[i], r, 1.0 + 2.0 * u[3 + i])) @ scale(r)
         refl = tuple(float(0.2 + 0.6 * u[6 + 3 * i + k]) for k in range(3))
         meshes.append(MeshData(sv, sf, object_to_world=o2w.astype(np.float32), refl=refl))
+    cam = Camera(look_at((0.0, 2.2, -6.0), (0.0, 0.8, 1.5)), 60.0, width, height)
+    return meshes, cam, (180.0, 200.0, 230.0)
+
+
+def noise_texture(width: int, height: int, seed: int) -> Texture:
+    """Seeded RGBA8 texture: 8x8-texel blocks of one colour each plus a per-texel
+    jitter of up to 31, alpha 255."""
+    rng = Rng(seed)
+    bw, bh = (width + 7) // 8, (height + 7) // 8
+    base = (rng.uniform(bw * bh * 3) * 224.0).astype(np.uint32).reshape(bh, bw, 3)
+    jit = (rng.uniform(width * height * 3) * 32.0).astype(np.uint32).reshape(height, width, 3)
+    rgb = np.repeat(np.repeat(base, 8, 0), 8, 1)[:height, :width] + jit
+    rgba = np.concatenate([rgb, np.full((height, width, 1), 255, np.uint32)], 2)
+    return Texture(rgba.astype(np.uint8), f"noise{seed}")
+
+
+def latlong_uv(n_lat: int, n_lon: int) -> np.ndarray:
+    """Texture coordinates for latlong_sphere's vertices: u = j / n_lon around,
+    v = 1 - i / n_lat from the north ring down."""
+    i = np.repeat(np.arange(n_lat + 1, dtype=np.float64), n_lon)
+    j = np.tile(np.arange(n_lon, dtype=np.float64), n_lat + 1)
+    return np.stack([j / n_lon, 1.0 - i / n_lat], 1).astype(np.float32)
+
+
+def textured_render_scene(seed: int = 1, width: int = 640, height: int = 480):
+    """plain_render_scene's layout with textures: a ground quad whose texture
+    coordinates run from -0.25 to 2.25 (texels past the image's edge: the next
+    row's first texel where that index stays inside the image, else the edge),
+    a textured lat-long sphere, a constant-Kd sphere and a mirror sphere; two
+    seeded noise textures (61 x 37 and 256 x 128 texels)."""
+    rng = Rng(seed)
+    u = rng.uniform(16)
+    floor_tex = noise_texture(61, 37, seed + 10)
+    ball_tex = noise_texture(256, 128, seed + 11)
+    gv, gf = quad(12.0, 0.0)
+    guv = np.array([[-0.25, -0.25], [2.25, -0.25], [2.25, 2.25], [-0.25, 2.25]], np.float32)
+    meshes = [MeshData(gv, gf, texture=floor_tex, uv=guv)]
+    sv, sf = latlong_sphere(20, 40, 1.0)
+    suv = latlong_uv(20, 40)
+    xs = (-2.6, 0.0, 2.6)
+    for i in range(3):
+        r = 0.5 + u[i]
+        o2w = (translate((xs[i], r, 1.0 + 2.0 * u[3 + i])) @ scale(r)).astype(np.float32)
+        if i == 0:
+            meshes.append(MeshData(sv, sf, object_to_world=o2w, texture=ball_tex, uv=suv))
+        elif i == 1:
+            meshes.append(MeshData(sv, sf, object_to_world=o2w, brdf=MIRROR, refl=(0.9, 0.85, 0.8)))
+        else:
+            meshes.append(MeshData(sv, sf, object_to_world=o2w, refl=tuple(float(0.2 + 0.6 * u[6 + k])
+                                                                                for k in range(3))))
     cam = Camera(look_at((0.0, 2.2, -6.0), (0.0, 0.8, 1.5)), 60.0, width, height)
     return meshes, cam, (180.0, 200.0, 230.0)
 

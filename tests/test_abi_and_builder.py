@@ -36,7 +36,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header():
-    assert C.sizeof(_abi.MeshDesc) == 256          # ... emission[3], specular[3], roughness
+    assert C.sizeof(_abi.MeshDesc) == 280          # ... specular[3], roughness, u, v, num_uv, texture
+    assert _abi.MeshDesc.u.offset == 256 and _abi.MeshDesc.texture.offset == 276
+    assert C.sizeof(_abi.TextureDesc) == 16
+    assert _abi.SceneDesc.textures.offset == 80 and C.sizeof(_abi.SceneDesc) == 96
     assert C.sizeof(_abi.BvhNode) == 32
     assert C.sizeof(_abi.CameraDesc) == 16 * 4 + 4 + 8 + 16
     assert _abi.HIT_DTYPE.itemsize == 24

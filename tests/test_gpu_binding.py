@@ -58,3 +58,21 @@ def test_gpu_runner_lambert_equals_oracle(tmp_path):
     film = run_check(tmp_path, meshes, cam, 3, 5, sky, 7)
     ref = O.OracleScene(meshes, sky).render(cam, 3, 5, seed=7)
     assert np.array_equal(film.view(np.uint32), ref.view(np.uint32))
+
+
+def test_gpu_runner_textured_scene_file_equals_oracle(tmp_path):
+    """A scene file with PNG textures and PLY u/v read by the reference's own
+    parser (lodepng decode, texture_directive, PLY u/v) and rendered through
+    GpuRunner: the oracle's film of the same file read by polytope_amd.pbrt."""
+    from polytope_amd import pbrt as P
+    from tests.test_textures import TEXTURED, _crafted
+    _crafted(tmp_path)
+    (tmp_path / "s.pbrt").write_text(TEXTURED)
+    film_path = tmp_path / "film.bin"
+    args = [str(O.GPU_RUNNER_CHECK), "--pbrt", str(tmp_path / "s.pbrt"), "--out", str(film_path), "--seed", "4"]
+    print("case", " ".join(args[1:]), flush=True)
+    subprocess.run(args, check=True, timeout=300)
+    sf = P.read_pbrt(str(tmp_path / "s.pbrt"))
+    film = np.fromfile(film_path, np.float32).reshape(sf.camera.height, sf.camera.width, 3)
+    ref = O.OracleScene(sf.meshes, sf.skybox).render(sf.camera, sf.spp, sf.max_depth, seed=4)
+    assert np.array_equal(film.view(np.uint32), ref.view(np.uint32))

@@ -48,7 +48,7 @@ def _assert_same(ref, sf):
             assert rm["spd"] == _bits(m.emission), i
         elif i not in unrenderable:
             assert rm["brdf"] == {S.LAMBERT: "lambert", S.MIRROR: "mirror", S.GLOSSY: "glossy"}[m.brdf], i
-            if m.brdf != S.GLOSSY:
+            if m.brdf != S.GLOSSY and m.texture is None:
                 assert rm["refl"] == _bits(m.refl), i
         for j, axis in enumerate("xyz"):
             assert rm[axis] == _bits(m.verts[:, j]), (i, axis)
@@ -60,6 +60,11 @@ def _assert_same(ref, sf):
                 assert rm[axis] == _bits(m.normals[:, j]), (i, axis)
         else:
             assert not rm["has_normals"] or not rm["nx"], i
+        if m.texture is not None:           # textured Lambert: texture size and coordinates
+            assert (rm["texture"]["width"], rm["texture"]["height"]) == (m.texture.width, m.texture.height), i
+            assert rm["u"] == _bits(m.uv[:, 0]) and rm["v"] == _bits(m.uv[:, 1]), i
+        elif i not in unrenderable and not m.is_light and m.brdf == S.LAMBERT:
+            assert "texture" not in rm, i
 
 
 # ----------------------------------------------------------------------------- crafted scenes
@@ -286,14 +291,17 @@ def test_point_lights_are_read_for_next_event_estimation(tmp_path):
 def test_unrenderable_meshes_are_refused_when_strict(tmp_path):
     _crafted_meshes(tmp_path)
     path = tmp_path / "s.pbrt"
+    from polytope_amd.png import encode_png
+    (tmp_path / "checks.png").write_bytes(encode_png(S.noise_texture(8, 8, 1).rgba))
     path.write_text('WorldBegin\nShape "plymesh" "string filename" [ "ico.ply" ]\n'
                     'Material "plastic" "rgb Kd" [ .5 .5 .5 ]\nShape "plymesh" "string filename" [ "ico.ply" ]\n'
+                    'Texture "checks" "color" "imagemap" "string filename" "checks.png"\n'
                     'Material "matte" "texture Kd" "checks"\nShape "plymesh" "string filename" [ "ico.ply" ]\n'
                     'WorldEnd\n')
     with pytest.raises(P.PbrtError, match="no material"):
         P.read_pbrt(str(path))
     sf = P.read_pbrt(str(path), strict=False)
-    assert [i for i, _ in sf.unrenderable] == [0, 2] and "textured" in sf.unrenderable[1][1]
+    assert [i for i, _ in sf.unrenderable] == [0, 2] and "texture coordinates" in sf.unrenderable[1][1]
     g = sf.meshes[1]                     # plastic: GlossyBRDF, Ks and roughness at their defaults
     assert g.brdf == S.GLOSSY and g.refl == (0.5, 0.5, 0.5)
     assert g.specular == tuple(float(np.float32(0.25)) for _ in range(3)) and g.roughness == float(np.float32(0.1))
