@@ -186,6 +186,9 @@ const char *ptx_last_error(void);
 const char *ptx_version(void);
 
 int ptx_device_open(int ordinal, ptx_device **out);
+/* Ends the handle: no new scene, query, render or server on it.  Scenes and
+ * servers made on it stay valid until destroyed (they keep the device state
+ * they need); a second close is a no-op. */
 void ptx_device_close(ptx_device *dev);
 
 int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **out);

@@ -48,6 +48,18 @@ int upload(ptx_scene *s, const std::vector<T> &v, const T **dst) {
 static_assert(sizeof(ptx_mesh_desc) == 280, "ptx_mesh_desc layout (Python and oracle mirrors)");
 static_assert(sizeof(ptx_texture_desc) == 16, "ptx_texture_desc layout (Python mirror)");
 
+void device_retain(ptx_device *dev) { dev->refs.fetch_add(1); }
+
+void device_release(ptx_device *dev) {
+   if (dev->refs.fetch_sub(1) == 1) delete dev;
+}
+
+int device_usable(const ptx_device *dev) {
+   if (!dev) return set_error(PTX_ERR_INVALID, "NULL device");
+   if (dev->closed.load()) return set_error(PTX_ERR_INVALID, "the device has been closed");
+   return PTX_OK;
+}
+
 uint32_t lds_depth_of(const ptx_device *dev) { return dev->tuning.lds_stack ? dev->tuning.lds_stack : 12; }
 
 uint32_t leaf_lanes_of(const ptx_device *dev, bool render) {
@@ -127,17 +139,21 @@ int ptx_device_open(int ordinal, ptx_device **out) {
 }
 
 void ptx_device_close(ptx_device *dev) {
-   if (!dev) return;
+   if (!dev || dev->closed.exchange(true)) return;
    (void)hipSetDevice(dev->ordinal);
-   dev->engine.reset();
-   if (dev->stream) (void)hipStreamDestroy(dev->stream);
-   dev->stream = nullptr;
-   delete dev;
+   {
+      std::lock_guard<std::mutex> lock(dev->mu);
+      dev->engine.reset();
+      if (dev->stream) (void)hipStreamDestroy(dev->stream);
+      dev->stream = nullptr;
+   }
+   device_release(dev);      // scenes and servers still alive keep the handle until they go
 }
 
 int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **out) {
    if (!dev || !desc || !out) return set_error(PTX_ERR_INVALID, "NULL argument");
    *out = nullptr;
+   if (int rc = device_usable(dev)) return rc;
    if (desc->num_meshes == 0 || !desc->meshes) return set_error(PTX_ERR_INVALID, "scene has no meshes");
    if (desc->num_lights && !desc->lights) return set_error(PTX_ERR_INVALID, "num_lights > 0 with NULL lights");
    if (desc->num_lights > 64) return set_error(PTX_ERR_INVALID, "at most 64 point lights");
@@ -244,15 +260,19 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    s->info.max_depth = img.max_stack;
    s->info.build_seconds = build_s;
    s->info.upload_seconds = now_s() - t1;
+   device_retain(dev);
    *out = s.release();
    return PTX_OK;
 }
 
 void ptx_scene_destroy(ptx_scene *scene) {
    if (!scene) return;
-   (void)hipSetDevice(scene->dev->ordinal);
+   ptx_device *dev = scene->dev;
+   (void)hipSetDevice(dev->ordinal);
    for (void *p : scene->allocations) (void)hipFree(p);
+   scene->allocations.clear();
    delete scene;
+   device_release(dev);
 }
 
 int ptx_scene_get_info(const ptx_scene *scene, ptx_scene_info *out) {
@@ -293,6 +313,7 @@ int ptx_bvh_build(const ptx_mesh_desc *meshes, uint32_t num_meshes, uint32_t thr
 int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t n, ptx_hit *d_hits,
                                  ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, void *stream) {
    if (!scene || (n && (!d_rays || !d_hits))) return set_error(PTX_ERR_INVALID, "NULL argument");
+   if (int rc = device_usable(scene->dev)) return rc;
    PTX_HIP_TRY(hipSetDevice(scene->dev->ordinal));
    return run_closest_hit(scene, d_rays, n, d_hits, d_counters, d_frames, static_cast<hipStream_t>(stream));
 }
@@ -300,6 +321,7 @@ int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t
 int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_hit *hits,
                           ptx_ray_counters *counters) {
    if (!scene || (n && (!rays || !hits))) return set_error(PTX_ERR_INVALID, "NULL argument");
+   if (int rc = device_usable(scene->dev)) return rc;
    if (n == 0) return PTX_OK;
    ptx_device *dev = scene->dev;
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
@@ -327,6 +349,7 @@ int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_h
 
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
    if (!dev || !t) return set_error(PTX_ERR_INVALID, "NULL argument");
+   if (int rc = device_usable(dev)) return rc;
    if (t->lds_stack && t->lds_stack != 8 && t->lds_stack != 12 && t->lds_stack != 16 && t->lds_stack != 24)
       return set_error(PTX_ERR_INVALID, "lds_stack must be 0, 8, 12, 16 or 24");
    if (t->render_mode > PTX_RENDER_WAVEFRONT) return set_error(PTX_ERR_INVALID, "render_mode must be 0, 1 or 2");

@@ -396,6 +396,7 @@ int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb) {
    int rc = validate_job(job);
    if (rc) return rc;
    ptx_device *dev = scene->dev;
+   if ((rc = device_usable(dev))) return rc;
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    std::lock_guard<std::mutex> lock(dev->mu);
    return engine_of(dev)->run_plain(scene, *job, film_rgb);
@@ -405,6 +406,7 @@ int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_
    if (!scenes || !jobs || !films || n_jobs == 0) return set_error(PTX_ERR_INVALID, "NULL argument or no jobs");
    ptx_device *dev = scenes[0] ? scenes[0]->dev : nullptr;
    if (!dev) return set_error(PTX_ERR_INVALID, "NULL scene");
+   if (int rc = device_usable(dev)) return rc;
    std::vector<const ptx_scene *> uniq;
    std::map<const ptx_scene *, uint32_t> index;
    std::vector<RenderJob> rj(n_jobs);
@@ -454,6 +456,7 @@ void look_at(const float eye[3], const float target[3], float m[16]) {
 
 extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chosen) {
    if (!dev || !scene) return set_error(PTX_ERR_INVALID, "NULL argument");
+   if (int rc = device_usable(dev)) return rc;
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    const ptx_tuning saved = dev->tuning;
    ptx_tuning best = saved;

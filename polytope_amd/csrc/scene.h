@@ -9,11 +9,15 @@
 #include "launch.h"
 #include "ptx_internal.h"
 
+// On failure: clears the calling thread's HIP error state (so that the caller's
+// own later HIP calls do not report it again) and returns PTX_ERR_HIP.
 #define PTX_HIP_TRY(expr)                                                                           \
    do {                                                                                             \
       hipError_t e_ = (expr);                                                                       \
-      if (e_ != hipSuccess)                                                                         \
+      if (e_ != hipSuccess) {                                                                       \
+         (void)hipGetLastError();                                                                   \
          return ::ptx::set_error(PTX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+      }                                                                                             \
    } while (0)
 
 struct ptx_device;
@@ -51,10 +55,18 @@ uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds, bool extra);
 size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds);
 int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
                     ptx_hit_frame *d_frames, hipStream_t stream);
+void device_retain(ptx_device *dev);
+void device_release(ptx_device *dev);
+// PTX_ERR_INVALID with a message when dev is NULL or closed, else PTX_OK.
+int device_usable(const ptx_device *dev);
 
 }  // namespace ptx
 
+// A device handle lives until ptx_device_close and the destruction of every
+// scene and server made on it, whichever comes last: each holds a reference.
 struct ptx_device {
+   std::atomic<int> refs{1};
+   std::atomic<bool> closed{false};  // ptx_device_close was called: no new work
    int ordinal = 0;
    int num_cus = 0;
    std::string arch;

@@ -249,6 +249,7 @@ extern "C" {
 int ptx_server_create(ptx_device *dev, const ptx_server_config *cfg, ptx_server **out) {
    if (!dev || !cfg || !out) return set_error(PTX_ERR_INVALID, "NULL argument");
    *out = nullptr;
+   if (int rc = device_usable(dev)) return rc;
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    auto s = std::make_unique<ptx_server>();
    s->dev = dev;
@@ -279,6 +280,7 @@ int ptx_server_create(ptx_device *dev, const ptx_server_config *cfg, ptx_server 
    }
    s->slot_used.assign(kMaxServerJobs, false);
    s->t0 = now_s();
+   device_retain(dev);
    ptx_server *raw = s.release();
    raw->worker = std::thread([raw] { raw->loop(); });
    *out = raw;
@@ -298,7 +300,9 @@ void ptx_server_destroy(ptx_server *srv) {
    for (auto &j : srv->running) srv->eng.free_job(j->jd);
    srv->eng.release();
    (void)hipStreamDestroy(srv->stream);
+   ptx_device *dev = srv->dev;
    delete srv;
+   device_release(dev);
 }
 
 int ptx_server_submit(ptx_server *srv, ptx_scene *scene, const ptx_job_desc *job, uint64_t *job_id) {

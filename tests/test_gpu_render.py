@@ -187,3 +187,27 @@ def test_next_event_jobs_mixed_with_plain_jobs(dev):
             render_batch([(sc, cam, 1, 2, 5, True)])
     finally:
         dev.set_tuning()
+
+
+def test_scene_and_server_outlive_their_device_handle():
+    """Closing a device handle while scenes and a server made on it live: the
+    scene refuses new work with an error and is destroyed cleanly afterwards,
+    the server still finishes its job, and no HIP error is left for the
+    caller's own GPU work."""
+    from polytope_amd import PtxError, Server
+    print("case device closed before its scene and server", flush=True)
+    d = Device(0)
+    meshes, cam, sky = S.plain_render_scene(1, 24, 18)
+    sc = Scene(d, meshes, sky)
+    srv = Server(d, hbm_budget_bytes=1 << 30, paths_per_wave=1 << 14)
+    jid = srv.submit(sc, cam, 2, 3, 9)
+    d.close()
+    with pytest.raises(PtxError, match="closed"):
+        render(sc, cam, 1, 1, 1)
+    film, st = srv.wait(jid)
+    assert np.array_equal(film.view(np.uint32), O.OracleScene(meshes, sky).render(cam, 2, 3, 9).view(np.uint32))
+    srv.close()
+    sc.close()
+    x = torch.ones(4, device="cuda") * 2
+    torch.cuda.synchronize()
+    assert float(x.sum()) == 8.0

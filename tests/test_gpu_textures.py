@@ -38,7 +38,7 @@ def test_textured_all_forms_match_oracle(dev, spp, depth):
     print(f"case textured scene spp={spp} depth={depth}", flush=True)
     meshes, cam, sky = S.textured_render_scene(2, 80, 60)
     ref = O.OracleScene(meshes, sky).render(cam, spp, depth, seed=11)
-    assert len(np.unique(ref.reshape(-1, 3), axis=0)) > 1000
+    assert len(np.unique(ref.reshape(-1, 3), axis=0)) > 500          # textures visible
     sc = Scene(dev, meshes, sky)
     assert _same(render(sc, cam, spp, depth, 11), ref)
     for mode in (_abi.RENDER_REGEN, _abi.RENDER_WAVEFRONT):
