@@ -840,8 +840,8 @@ struct NeeRecord {
    float4 a, b, c;   // a = P.xyz, wout.x; b = wout.yz, k.xy; c = k.z, n.xyz
 };
 
-// kMat: some scene of the round has glossy or textured meshes.
-template <int kLds, bool kNee, bool kMat>
+// kGlossy: some scene of the round has glossy meshes; kTex: some has textured ones.
+template <int kLds, bool kNee, bool kGlossy, bool kTex>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
         uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
@@ -920,10 +920,10 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
       }
       const uint32_t m0 = mode & kPModeMask;
       if (!__any(m0 != kPIdle)) break;
-      const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3 || (kMat && m0 == kPShade4);
+      const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3 || (kTex && m0 == kPShade4);
       const bool tracing = m0 == kPTrace || m0 == kPShadow;
       // single-load steps that never wait for the wave
-      const bool light_step = m0 == kPShade1 || m0 == kPLight || (kMat && m0 >= kPTex1 && m0 <= kPTex3);
+      const bool light_step = m0 == kPShade1 || m0 == kPLight || (kTex && m0 >= kPTex1 && m0 <= kPTex3);
       const bool run_heavy = (uint32_t)__popcll(__ballot(heavy)) >= shade_min || !__any(tracing || light_step);
       // leaf steps gather the same way; lanes at interior nodes, in Shade1 or in
       // Light always step, so some lane progresses whenever any lane holds
@@ -948,7 +948,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          f.q[12] = q3.x; f.q[13] = q3.y; f.q[14] = q3.z; f.q[15] = q3.w;
       } else if (m == kPShade1) {
          const PTX_GLOBAL uint32_t *pp = as_global(ssc[sch[lw_chunk(mode)].scene].pairs) + 2 * (size_t)s.hit;
-         if (kMat) {                                // (mesh, face)
+         if (kTex) {                                // (mesh, face)
             const u32x2 pr = *reinterpret_cast<const PTX_GLOBAL u32x2 *>(pp);
             f.q[0] = __uint_as_float(pr.x);
             f.q[1] = __uint_as_float(pr.y);
@@ -968,21 +968,21 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          const PTX_GLOBAL float *nrm = as_global(ssc[sch[lw_chunk(mode)].scene].normals + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) { f.q[k] = v[k]; f.q[9 + k] = nrm[k]; }
-      } else if (kMat && m == kPShade4) {
+      } else if (kTex && m == kPShade4) {
          const PTX_GLOBAL float *v = as_global(tris + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) f.q[k] = v[k];
-      } else if (kMat && m == kPTex1) {            // MeshRec words 9-13: tex_wh, tex_uv, texels
+      } else if (kTex && m == kPTex1) {            // MeshRec words 9-13: tex_wh, tex_uv, texels
          const PTX_GLOBAL uint32_t *mr =
                as_global(reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0));
 #pragma unroll
          for (int k = 0; k < 5; k++) f.q[k] = __uint_as_float(mr[9 + k]);
-      } else if (kMat && m == kPTex2) {            // the face's corner coordinates
+      } else if (kTex && m == kPTex2) {            // the face's corner coordinates
          const PTX_GLOBAL float *uv = as_global(reinterpret_cast<const float *>(
                (uint64_t)__float_as_uint(s.r.ix) | ((uint64_t)__float_as_uint(s.r.iy) << 32)));
 #pragma unroll
          for (int k = 0; k < 6; k++) f.q[k] = uv[k];
-      } else if (kMat && m == kPTex3) {            // the texel
+      } else if (kTex && m == kPTex3) {            // the texel
          const PTX_GLOBAL uint32_t *tx = as_global(reinterpret_cast<const uint32_t *>(
                (uint64_t)__float_as_uint(s.r.ix) | ((uint64_t)__float_as_uint(s.r.iy) << 32)));
          f.q[0] = __uint_as_float(*tx);
@@ -1064,7 +1064,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          } else {
             const DevScene &S = ssc[c.scene];
             MeshRec M;
-            if (kMat && (w0 & 2u)) {               // textured Lambert: the texel rides in the record
+            if (kTex && (w0 & 2u)) {               // textured Lambert: the texel rides in the record
                M.brdf = PTX_BRDF_LAMBERT;
                texel_refl(w0 >> 8, M.refl);
             } else {
@@ -1073,7 +1073,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
                M.refl[0] = __uint_as_float(mw[1]);
                M.refl[1] = __uint_as_float(mw[2]);
                M.refl[2] = __uint_as_float(mw[3]);
-               if (kMat) M.emission[0] = __uint_as_float(mw[5]);       // glossy roughness
+               if (kGlossy) M.emission[0] = __uint_as_float(mw[5]);       // glossy roughness
             }
             Frame fr;
             fr.lx = f.q[1]; fr.ly = f.q[2]; fr.lz = f.q[3];
@@ -1088,7 +1088,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const float kd[3] = {M.refl[0], M.refl[1], M.refl[2]};
             const float s_before[3] = {src[0], src[1], src[2]};
             float wout[3];
-            bounce<kMat>(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
+            bounce<kGlossy>(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
             nodes = S.nodes;
             tris = S.tris;
             if (nee) {
@@ -1108,9 +1108,9 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          }
       } else if (m == kPShade1) {
          t0 = __float_as_uint(f.q[0]);
-         if (kMat) s.r.ox = f.q[1];                // the hit's face, for a textured mesh
+         if (kTex) s.r.ox = f.q[1];                // the hit's face, for a textured mesh
          mode = lw_with_state(mode, kPShade2);
-      } else if (kMat && m == kPTex1) {
+      } else if (kTex && m == kPTex1) {
          const uint32_t face = __float_as_uint(s.r.ox);
          const uint64_t uv = ((uint64_t)__float_as_uint(f.q[1]) | ((uint64_t)__float_as_uint(f.q[2]) << 32)) +
                              (uint64_t)face * 24u;
@@ -1120,21 +1120,21 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          s.r.iz = f.q[3];                          // texels
          s.r.Sx = f.q[4];
          mode = lw_with_state(mode, kPTex2);
-      } else if (kMat && m == kPTex2) {
+      } else if (kTex && m == kPTex2) {
          const uint32_t k = texel_index(&f.q[0], s.hu, s.hv, s.hw, __float_as_uint(s.r.oz));
          const uint64_t tx = ((uint64_t)__float_as_uint(s.r.iz) | ((uint64_t)__float_as_uint(s.r.Sx) << 32)) +
                              (uint64_t)k * 4u;
          s.r.ix = __uint_as_float((uint32_t)tx);
          s.r.iy = __uint_as_float((uint32_t)(tx >> 32));
          mode = lw_with_state(mode, kPTex3);
-      } else if (kMat && m == kPTex3) {
+      } else if (kTex && m == kPTex3) {
          float refl[3];
          texel_refl(__float_as_uint(f.q[0]), refl);
          t0 = (uint32_t)PTX_BRDF_LAMBERT;
          t1 = refl[0]; t2 = refl[1]; t3 = refl[2];
          const bool normals = (__float_as_uint(s.r.oy) & kRecNormals) && ssc[sch[lw_chunk(mode)].scene].normals;
          mode = lw_with_state(mode, normals ? kPShade3 : kPShade4);
-      } else if (m == kPShade2 || m == kPShade3 || (kMat && m == kPShade4)) {
+      } else if (m == kPShade2 || m == kPShade3 || (kTex && m == kPShade4)) {
          const LdsChunk &c = sch[lw_chunk(mode)];
          const uint32_t b = lw_bounce(mode);
          bool go = true;
@@ -1142,7 +1142,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const int32_t brdf = __float_as_int(f.q[0]);
             const int32_t is_light = __float_as_int(f.q[4]);
             const uint32_t mflags = __float_as_uint(f.q[8]);
-            if (kMat) s.cnt = __float_as_uint(f.q[5]);             // glossy roughness, kept until the bounce
+            if (kGlossy) s.cnt = __float_as_uint(f.q[5]);             // glossy roughness, kept until the bounce
             if (is_light) {
                write_res(res, slot, fin<kNee>(Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
                                                        (f.q[7] + 0.f) * src[2]}, direct, mode));
@@ -1152,7 +1152,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
                write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
                mode = kPIdle;
                go = false;
-            } else if (kMat && (mflags & kRecTextured)) {     // t0 still holds the mesh
+            } else if (kTex && (mflags & kRecTextured)) {     // t0 still holds the mesh
                s.r.oy = f.q[8];
                mode = lw_with_state(mode, kPTex1);
                go = false;
@@ -1172,7 +1172,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             MeshRec M;
             M.brdf = (int32_t)t0;
             M.refl[0] = t1; M.refl[1] = t2; M.refl[2] = t3;
-            if (kMat) M.emission[0] = __uint_as_float(s.cnt);
+            if (kGlossy) M.emission[0] = __uint_as_float(s.cnt);
             Frame fr;
             hit_frame_sel(&f.q[0], &f.q[9], m == kPShade3, s.r, s.hu, s.hv, s.hw, s.after, fr);
             const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
@@ -1183,7 +1183,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const float s_before[3] = {src[0], src[1], src[2]};
             float lin[3], wout[3];
             world_to_local(fr, d, lin);
-            bounce<kMat>(M, fr, lin, pixel, sample, b, c.seed, wout, src);
+            bounce<kGlossy>(M, fr, lin, pixel, sample, b, c.seed, wout, src);
             if (nee) {
                const float keep[3] = {src[0], src[1], src[2]};
                src[0] = s_before[0]; src[1] = s_before[1]; src[2] = s_before[2];
@@ -1265,14 +1265,21 @@ void launch_zero_floats(float *a, size_t n, hipStream_t st) {
 }
 
 int paths_blocks_per_cu() {
-   int b[4] = {0, 0, 0, 0};
-   hipError_t e[4];
-   e[0] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[0], dev::k_paths<kRenderLds, false, false>, 256, 0);
-   e[1] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[1], dev::k_paths<kRenderLds, true, false>, 256, 0);
-   e[2] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[2], dev::k_paths<kRenderLds, false, true>, 256, 0);
-   e[3] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[3], dev::k_paths<kRenderLds, true, true>, 256, 0);
+   int b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+   hipError_t e[8];
+#define PTX_PATHS_OCC(I, N, G, T) \
+   e[I] = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b[I], dev::k_paths<kRenderLds, N, G, T>, 256, 0)
+   PTX_PATHS_OCC(0, false, false, false);
+   PTX_PATHS_OCC(1, true, false, false);
+   PTX_PATHS_OCC(2, false, true, false);
+   PTX_PATHS_OCC(3, true, true, false);
+   PTX_PATHS_OCC(4, false, false, true);
+   PTX_PATHS_OCC(5, true, false, true);
+   PTX_PATHS_OCC(6, false, true, true);
+   PTX_PATHS_OCC(7, true, true, true);
+#undef PTX_PATHS_OCC
    int m = b[0];
-   for (int i = 0; i < 4; i++) {
+   for (int i = 0; i < 8; i++) {
       if (e[i] != hipSuccess) return 1;
       m = std::min(m, b[i]);
    }
@@ -1284,15 +1291,22 @@ uint32_t paths_max_scenes() { return dev::kMaxRoundScenes; }
 
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
-                  uint32_t leaf_min, float4 *nee_scratch, bool materials, uint32_t grid, hipStream_t st) {
+                  uint32_t leaf_min, float4 *nee_scratch, bool glossy, bool textured, uint32_t grid, hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
-#define PTX_LAUNCH_PATHS(N, G)                                                                                     \
-   hipLaunchKernelGGL((dev::k_paths<kRenderLds, N, G>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, \
+#define PTX_LAUNCH_PATHS(N, G, T)                                                                                     \
+   hipLaunchKernelGGL((dev::k_paths<kRenderLds, N, G, T>), dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, \
                       scenes, nscenes, res, work, spill, max_stack, shade_min, leaf_min, nee_scratch)
-   if (nee_scratch && materials) PTX_LAUNCH_PATHS(true, true);
-   else if (nee_scratch) PTX_LAUNCH_PATHS(true, false);
-   else if (materials) PTX_LAUNCH_PATHS(false, true);
-   else PTX_LAUNCH_PATHS(false, false);
+   const int v = (nee_scratch ? 1 : 0) | (glossy ? 2 : 0) | (textured ? 4 : 0);
+   switch (v) {
+      case 0: PTX_LAUNCH_PATHS(false, false, false); break;
+      case 1: PTX_LAUNCH_PATHS(true, false, false); break;
+      case 2: PTX_LAUNCH_PATHS(false, true, false); break;
+      case 3: PTX_LAUNCH_PATHS(true, true, false); break;
+      case 4: PTX_LAUNCH_PATHS(false, false, true); break;
+      case 5: PTX_LAUNCH_PATHS(true, false, true); break;
+      case 6: PTX_LAUNCH_PATHS(false, true, true); break;
+      default: PTX_LAUNCH_PATHS(true, true, true); break;
+   }
 #undef PTX_LAUNCH_PATHS
 }
 

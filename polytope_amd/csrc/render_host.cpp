@@ -241,7 +241,7 @@ int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
 }
 
 int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes,
-                       bool nee, bool materials) {
+                       bool nee, bool glossy, bool textured) {
    uint32_t nslots = 0;
    for (const Chunk &c : r) nslots = std::max(nslots, c.base + c.k * c.npix);
    if (nee && !regen_round((uint32_t)r.size(), nscenes))
@@ -253,7 +253,7 @@ int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint3
    if (regen_round((uint32_t)r.size(), nscenes)) {
       const uint32_t shade_min = dev->tuning.shade_lanes ? dev->tuning.shade_lanes : kDefaultShadeLanes;
       launch_paths(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, nscenes, res, words + kWork, spill, stack_cap,
-                   shade_min, leaf_lanes_of(dev, true), nee ? nee_scratch : nullptr, materials,
+                   shade_min, leaf_lanes_of(dev, true), nee ? nee_scratch : nullptr, glossy, textured,
                    std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
       launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
       return PTX_OK;
@@ -328,14 +328,14 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
    size_t off = 0;
    for (const auto &r : rounds) {
       uint32_t depth = 0;
-      bool nee = false, materials = false;
+      bool nee = false, glossy = false, textured = false;
       for (const Chunk &c : r) {
          depth = std::max(depth, jobs[c.job].desc.max_depth);
          nee |= (jobs[c.job].desc.flags & PTX_JOB_NEE) != 0;
-         const ptx_scene *sc = scenes[jobs[c.job].scene_index];
-         materials |= sc->has_glossy || sc->has_texture;
+         glossy |= scenes[jobs[c.job].scene_index]->has_glossy;
+         textured |= scenes[jobs[c.job].scene_index]->has_texture;
       }
-      if ((rc = round(d_chunks + off, r, depth, (uint32_t)scenes.size(), nee, materials))) {
+      if ((rc = round(d_chunks + off, r, depth, (uint32_t)scenes.size(), nee, glossy, textured))) {
          (void)hipStreamSynchronize(st);
          cleanup();
          return rc;

@@ -70,9 +70,9 @@ struct RenderEngine {
    // One round over the chunks in d_round (a device copy of r); the job and
    // scene tables must hold the round's jobs, the scene table nscenes entries.
    // nee: some job of the round uses next-event estimation (regeneration only);
-   // materials: some scene of the round has glossy or textured meshes.
+   // glossy / textured: some scene of the round has glossy / textured meshes.
    int round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes, bool nee,
-             bool materials);
+             bool glossy, bool textured);
    int finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host);
    void free_job(JobDev &J);
    uint32_t capacity_for(uint64_t total_paths, uint32_t max_npix, bool regen) const;

@@ -182,7 +182,7 @@ void ptx_server::loop() {
          chunks.clear();
          uint64_t used = 0;
          uint32_t depth = 0;
-         bool nee = false, materials = false;
+         bool nee = false, glossy = false, textured = false;
          const uint64_t share = paths / in_round.size();
          for (auto &j : in_round) {
             const uint32_t left = j->desc.spp - j->next_sample;
@@ -194,12 +194,13 @@ void ptx_server::loop() {
             used += k * j->npix;
             depth = std::max(depth, j->desc.max_depth);
             nee |= (j->desc.flags & PTX_JOB_NEE) != 0;
-            materials |= j->scene->has_glossy || j->scene->has_texture;
+            glossy |= j->scene->has_glossy;
+            textured |= j->scene->has_texture;
          }
          if (!chunks.empty()) {
             (void)hipMemcpyAsync(eng.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice,
                                  stream);
-            round_rc = eng.round(eng.d_chunks, chunks, depth, kMaxServerScenes, nee, materials);
+            round_rc = eng.round(eng.d_chunks, chunks, depth, kMaxServerScenes, nee, glossy, textured);
          }
          const hipError_t e = hipStreamSynchronize(stream);
          std::lock_guard<std::mutex> lock(mu);
