@@ -157,7 +157,21 @@ def render_section(dev, args):
     print(f"[bench] plain: {samples / t_plain / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
     same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) and
                np.array_equal(a.view(np.uint32), w.view(np.uint32)) for a, b, w in zip(films_b, films_p, films_w))
+    # the same geometry and BVH with textured Lambert and glossy meshes
+    mat_scene = Scene(dev, S.with_materials(meshes), sky, bvh=scene.bvh())
     scene.close()
+    mat_batch = [(mat_scene, c, spp, d, seed) for c, spp, d, seed in jobs]
+    render_batch([(mat_scene, jobs[0][0], 1, 8, 1)])
+    tm = time.perf_counter()
+    films_m = render_batch(mat_batch)
+    t_mat = time.perf_counter() - tm
+    tmp = time.perf_counter()
+    films_mp = [render(mat_scene, c, spp, d, seed) for c, spp, d, seed in jobs]
+    t_mat_plain = time.perf_counter() - tmp
+    same_m = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(films_m, films_mp))
+    print(f"[bench] textured + glossy: batched {samples / t_mat / 1e6:.1f}, plain {samples / t_mat_plain / 1e6:.1f} "
+          f"Msamples/s", file=sys.stderr, flush=True)
+    mat_scene.close()
     return {"workload": f"icosphere subdiv {args.render_subdiv} x {args.render_instances + 1} instances + ground "
                         f"({sum(m.num_faces for m in meshes)} tris), 16 jobs 256x256..1920x1080, "
                         f"{args.render_spp} spp, depth 8, 70% Lambert / 30% mirror",
@@ -165,7 +179,11 @@ def render_section(dev, args):
             "batched_samples_per_s": round(samples / t_batched, 1), "plain_samples_per_s": round(samples / t_plain, 1),
             "batched_vs_plain": round(t_plain / t_batched, 3),
             "wavefront_samples_per_s": round(samples / t_wave, 1), "films_identical": same,
-            "bvh_build_s": round(info["build_seconds"], 2)}
+            "bvh_build_s": round(info["build_seconds"], 2),
+            "materials": {"workload": "the same meshes and BVH, Lambert meshes textured (512x512 RGBA8), mirrors "
+                                      "glossy (roughness 0.2)",
+                          "batched_samples_per_s": round(samples / t_mat, 1),
+                          "plain_samples_per_s": round(samples / t_mat_plain, 1), "films_identical": same_m}}
 
 
 def plain_cpu_render_section(dev, args, threads: int):

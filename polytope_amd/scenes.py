@@ -646,6 +646,39 @@ def textured_render_scene(seed: int = 1, width: int = 640, height: int = 480):
     return meshes, cam, (180.0, 200.0, 230.0)
 
 
+def planar_uv(verts: np.ndarray, repeat: float = 1.0) -> np.ndarray:
+    """Texture coordinates by projection onto the x-z plane of the vertices'
+    bounding box, scaled by ``repeat``."""
+    v = np.asarray(verts, np.float64)
+    lo, hi = v.min(0), v.max(0)
+    span = np.where(hi - lo > 0, hi - lo, 1.0)
+    return np.stack([(v[:, 0] - lo[0]) / span[0] * repeat, (v[:, 2] - lo[2]) / span[2] * repeat],
+                    1).astype(np.float32)
+
+
+def with_materials(meshes: Sequence[MeshData], seed: int = 5, texture_size: int = 512) -> List[MeshData]:
+    """The same geometry with the f4 materials: every Lambert mesh textured with
+    one seeded ``texture_size``^2 noise texture (planar coordinates, repeated 8
+    times on large meshes), every mirror turned into the glossy lobe (Kd its
+    reflectance, Ks 0.9, roughness 0.2)."""
+    tex = noise_texture(texture_size, texture_size, seed)
+    out = []
+    uv_cache = {}
+    for m in meshes:
+        if m.is_light:
+            out.append(m)
+        elif m.brdf == LAMBERT:
+            key = id(m.verts)
+            if key not in uv_cache:
+                uv_cache[key] = planar_uv(m.verts, 8.0 if m.num_faces <= 2 else 1.0)
+            out.append(MeshData(m.verts, m.faces, object_to_world=m.object_to_world, normals=m.normals,
+                                texture=tex, uv=uv_cache[key]))
+        else:
+            out.append(MeshData(m.verts, m.faces, object_to_world=m.object_to_world, normals=m.normals,
+                                brdf=GLOSSY, refl=m.refl, specular=(0.9, 0.9, 0.9), roughness=0.2))
+    return out
+
+
 def batched_render_workload(subdiv: int = 8, instances: int = 32, jobs: int = 16, spp: int = 64,
                             max_depth: int = 8, seed: int = 3):
     """Batched wavefront workload: a displaced icosphere (the shared geometry of
