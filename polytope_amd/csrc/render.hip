@@ -165,6 +165,10 @@ __device__ __forceinline__ bool glossy_sample(const float in[3], float rough, ui
                                               uint32_t bounce_index, uint32_t seed, float &lx, float &ly,
                                               float &lz) {
    for (uint32_t a = 0; a < kGlossyTries; a++) {
+      // the mirror frame below depends only on `in`; recomputing it per attempt
+      // keeps four values live across the loop instead of the whole frame
+      float i0 = in[0], i1 = in[1], i2 = in[2];
+      asm volatile("" : "+v"(i0), "+v"(i1), "+v"(i2));
       uint32_t r0, r1;
       philox4x32_10(pixel, sample, bounce_index, a, seed, kRngKeyHi, r0, r1);
       const float u0 = u01(r0), u1 = u01(r1);
@@ -174,8 +178,8 @@ __device__ __forceinline__ bool glossy_sample(const float in[3], float rough, ui
       sincos_poly(theta, sn, cs);
       const float om = 1.0f - u0;
       const float hx = r * cs, hy = __builtin_sqrtf(0.0f < om ? om : 0.0f), hz = r * sn;
-      const float factor = (0.f * in[0] + 1.f * in[1] + 0.f * in[2]) * 2;
-      float sx = in[0] - 0.f * factor, sy = in[1] - 1.f * factor, sz = in[2] - 0.f * factor;
+      const float factor = (0.f * i0 + 1.f * i1 + 0.f * i2) * 2;
+      float sx = i0 - 0.f * factor, sy = i1 - 1.f * factor, sz = i2 - 0.f * factor;
       normalize3(sx, sy, sz);
       const float angle = -acos_poly(0.f * sx + 1.f * sy + 0.f * sz);
       const float x = sy * 0.f - sz * 1.f, y = sz * 0.f - sx * 0.f, z = sx * 1.f - sy * 0.f;
