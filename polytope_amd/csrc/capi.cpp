@@ -215,27 +215,39 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    if ((rc = upload(s.get(), img.nodes, &s->d.nodes)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.tris, &s->d.tris)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), s->bvh.pairs, &s->d.pairs)) != PTX_OK) return rc;
+   if ((rc = upload(s.get(), img.meshes, &s->d.meshes)) != PTX_OK) return rc;
    if (s->has_texture) {
-      // one device copy per texture that a mesh uses, then each textured mesh's
-      // per-face corner coordinates; the mesh records point at both
-      std::vector<const uint32_t *> texels(desc->num_textures, nullptr);
+      // one texture record per leaf-ordered triangle, then the texel pool (each
+      // texture that a mesh uses, once), in one allocation
+      const size_t nt = img.num_tris;
+      std::vector<int64_t> first(desc->num_textures, -1);
+      size_t pool = 0;
       for (uint32_t i = 0; i < desc->num_meshes; i++) {
          const int32_t t = meshes[i].texture;
-         if (t < 0) continue;
-         const ptx_texture_desc &x = desc->textures[t];
-         if (!texels[t]) {
-            std::vector<uint32_t> px(size_t(x.width) * x.height);
-            std::memcpy(px.data(), x.rgba, px.size() * 4);
-            if ((rc = upload(s.get(), px, &texels[t])) != PTX_OK) return rc;
-         }
-         const float *uv = nullptr;
-         if ((rc = upload(s.get(), meshes[i].uv, &uv)) != PTX_OK) return rc;
-         img.meshes[i].texels = texels[t];
-         img.meshes[i].tex_uv = uv;
-         img.meshes[i].tex_wh = x.width | x.height << 16;
+         if (t < 0 || first[t] >= 0) continue;
+         first[t] = (int64_t)pool;
+         pool += size_t(desc->textures[t].width) * desc->textures[t].height;
       }
+      if (pool > 0xFFFFFFFFull) return set_error(PTX_ERR_INVALID, "a scene's textures exceed 2^32 texels");
+      std::vector<uint32_t> blob(nt * (sizeof(TexTri) / 4) + pool, 0u);
+      TexTri *rec = reinterpret_cast<TexTri *>(blob.data());
+      for (size_t k = 0; k < nt; k++) {
+         const uint32_t mi = s->bvh.pairs[2 * k], face = s->bvh.pairs[2 * k + 1];
+         const WorldMesh &m = meshes[mi];
+         if (m.texture < 0) continue;
+         const ptx_texture_desc &x = desc->textures[m.texture];
+         std::copy(&m.uv[6 * size_t(face)], &m.uv[6 * size_t(face)] + 6, rec[k].uv);
+         rec[k].wh = x.width | x.height << 15 | kTexTextured | (m.has_normals ? kTexNormals : 0u);
+         rec[k].texel0 = (uint32_t)first[m.texture];
+      }
+      for (uint32_t t = 0; t < desc->num_textures; t++)
+         if (first[t] >= 0)
+            std::memcpy(&blob[nt * (sizeof(TexTri) / 4) + first[t]], desc->textures[t].rgba,
+                        size_t(desc->textures[t].width) * desc->textures[t].height * 4);
+      const uint32_t *dev_blob = nullptr;
+      if ((rc = upload(s.get(), blob, &dev_blob)) != PTX_OK) return rc;
+      s->d.tex_tris = reinterpret_cast<const TexTri *>(dev_blob);
    }
-   if ((rc = upload(s.get(), img.meshes, &s->d.meshes)) != PTX_OK) return rc;
    if ((rc = upload(s.get(), img.normals, &s->d.normals)) != PTX_OK) return rc;
    if (desc->num_lights) {
       std::vector<float> lights(6 * (size_t)desc->num_lights);

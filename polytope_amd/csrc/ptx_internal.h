@@ -36,10 +36,10 @@ static_assert(sizeof(Node64) == 64, "Node64 is 64 bytes");
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 
-// Per-mesh shading record, 64 bytes (words 0-8 are what a shading step reads
-// first).  Glossy meshes that are not lights keep the lobe's constant
-// reflectance in refl and its roughness in emission[0].  A textured Lambert
-// mesh (kRecTextured) takes its reflectance from its texture instead of refl.
+// Per-mesh shading record (9 words, read as one block by the shading step).
+// Glossy meshes that are not lights keep the lobe's constant reflectance in
+// refl and its roughness in emission[0].  A textured Lambert mesh
+// (kRecTextured) takes its reflectance from its triangles' texture records.
 constexpr uint32_t kRecNormals = 1u, kRecTextured = 2u;
 struct MeshRec {
    int32_t brdf;
@@ -47,15 +47,21 @@ struct MeshRec {
    int32_t is_light;
    float emission[3];
    uint32_t flags;           // kRecNormals | kRecTextured
-   uint32_t tex_wh;          // textured: texture width | height << 16
-   const float *tex_uv;      // textured: (u, v) of each face's corners 0, 1, 2 (6 floats per face, face order)
-   const uint32_t *texels;   // textured: RGBA8 texels, rows top to bottom
-   uint32_t pad[2];
 };
-static_assert(sizeof(MeshRec) == 64, "MeshRec layout");
-static_assert(offsetof(MeshRec, flags) == 32 && offsetof(MeshRec, tex_wh) == 36 && offsetof(MeshRec, tex_uv) == 40 &&
-                    offsetof(MeshRec, texels) == 48,
-              "MeshRec word offsets");
+static_assert(sizeof(MeshRec) == 36, "MeshRec layout");
+
+// Texture record of one leaf-ordered triangle (32 bytes), present for every
+// triangle of a scene with textured meshes: its corner texture coordinates,
+// the texture's size and whether the triangle is textured, and the offset of
+// the texture's first texel in the scene's texel pool, which follows the
+// num_tris records in the same allocation.
+constexpr uint32_t kTexTextured = 1u << 31, kTexNormals = 1u << 30;
+struct TexTri {
+   float uv[6];              // (u, v) of corners 0, 1, 2 (fv0, fv1, fv2)
+   uint32_t wh;              // width | height << 15 | kTexNormals | kTexTextured
+   uint32_t texel0;          // first texel of the texture in the pool
+};
+static_assert(sizeof(TexTri) == 32, "TexTri layout");
 
 // Device view of one scene.  The first 48 bytes are what a traversal needs
 // (loaded per lane by the render path's extend stage as three 16-byte loads).
@@ -70,11 +76,11 @@ struct DevScene {
    const float *normals;     // world corner normals, 9 floats per leaf-ordered triangle (or null)
    uint32_t num_tris;
    uint32_t max_stack;       // deepest deferred-stack use + 1 (tree depth bound)
-   int32_t has_sky;
    float sky[3];
+   uint16_t num_lights;
+   uint16_t has_sky;
    const float *lights;      // point lights, 6 floats each (position, intensity), or null
-   uint32_t num_lights;
-   uint32_t pad;
+   const TexTri *tex_tris;   // texture records per leaf-ordered triangle, then the texel pool; or null
 };
 static_assert(sizeof(DevScene) == 112, "DevScene layout");
 

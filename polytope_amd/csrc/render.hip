@@ -315,7 +315,7 @@ __device__ __forceinline__ uint32_t texel_trunc(float f) {
 __device__ __forceinline__ uint32_t texel_index(const float uv[6], float u, float v, float w, uint32_t wh) {
    const float tu = (uv[0] * u + uv[2] * v) + uv[4] * w;
    const float tv = (uv[1] * u + uv[3] * v) + uv[5] * w;
-   const uint32_t W = wh & 0xFFFFu, H = wh >> 16;
+   const uint32_t W = wh & 0x7FFFu, H = (wh >> 15) & 0x7FFFu;
    const uint32_t col = texel_trunc(tu * (float)W), row = texel_trunc((1.0f - tv) * (float)H);
    const uint64_t flat = (uint64_t)row * W + col;
    if (flat < (uint64_t)W * H) return (uint32_t)flat;
@@ -329,11 +329,16 @@ __device__ __forceinline__ void texel_refl(uint32_t t, float refl[3]) {
    refl[2] = (float)((t >> 16) & 255u) * (1.0f / 255.0f);
 }
 
-// The RGBA8 texel under a hit on face `face` (mesh order) of textured mesh M.
-__device__ __forceinline__ uint32_t hit_texel(const MeshRec &M, uint32_t face, float u, float v, float w) {
-   const float *c = M.tex_uv + 6 * (size_t)face;
-   const float uv[6] = {c[0], c[1], c[2], c[3], c[4], c[5]};
-   return M.texels[texel_index(uv, u, v, w, M.tex_wh)];
+// The texel pool of a scene with texture records (after its num_tris records).
+__device__ __forceinline__ const uint32_t *texel_pool(const DevScene &S) {
+   return reinterpret_cast<const uint32_t *>(S.tex_tris + S.num_tris);
+}
+
+// The RGBA8 texel under a hit on leaf-ordered triangle `tri` of a textured mesh.
+__device__ __forceinline__ uint32_t tri_texel(const DevScene &S, uint32_t tri, float u, float v, float w) {
+   const TexTri &t = S.tex_tris[tri];
+   const float uv[6] = {t.uv[0], t.uv[1], t.uv[2], t.uv[3], t.uv[4], t.uv[5]};
+   return texel_pool(S)[t.texel0 + texel_index(uv, u, v, w, t.wh)];
 }
 
 // ---------------------------------------------------------------- primary hits
@@ -390,8 +395,7 @@ struct PrimaryPolicy {
             float lin[3];
             world_to_local(fr, d, lin);
             uint32_t w0 = mesh << 2;
-            if (M.flags & kRecTextured)
-               w0 = 2u | (hit_texel(M, S.pairs[2 * (size_t)s.hit + 1], s.hu, s.hv, s.hw) & 0xFFFFFFu) << 8;
+            if (M.flags & kRecTextured) w0 = 2u | (tri_texel(S, s.hit, s.hu, s.hv, s.hw) & 0xFFFFFFu) << 8;
             rec[0] = __uint_as_float(w0);
             rec[1] = fr.lx; rec[2] = fr.ly; rec[3] = fr.lz;
             rec[4] = fr.t1x; rec[5] = fr.t1y; rec[6] = fr.t1z;
@@ -611,8 +615,7 @@ __global__ void __launch_bounds__(256) k_shade(const float *queue_in, const uint
                write_res(res, path, Radiance{0.f, 0.f, 0.f});
             } else {
                MeshRec M = Mg;
-               if (M.flags & kRecTextured)
-                  texel_refl(hit_texel(M, S.pairs[2 * (size_t)tri + 1], h0.z, h0.w, h1.x), M.refl);
+               if (M.flags & kRecTextured) texel_refl(tri_texel(S, tri, h0.z, h0.w, h1.x), M.refl);
                const Frame fr = frame_of(S, tri, dir, h0.z, h0.w, h1.x, __float_as_uint(h1.y), mesh);
                float lin[3], wout[3];
                world_to_local(fr, dir, lin);
@@ -730,8 +733,7 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
          }
          if (b >= J.max_depth) break;
          MeshRec M = Mg;
-         if (kTex && (M.flags & kRecTextured))
-            texel_refl(hit_texel(M, S.pairs[2 * (size_t)s.hit + 1], s.hu, s.hv, s.hw), M.refl);
+         if (kTex && (M.flags & kRecTextured)) texel_refl(tri_texel(S, s.hit, s.hu, s.hv, s.hw), M.refl);
          const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
          if (nee && M.brdf == PTX_BRDF_LAMBERT) {
             for (uint32_t j = 0; j < S.num_lights; j++) {
@@ -784,10 +786,7 @@ enum : uint32_t {
    kPIdle = 0, kPPrim = 1, kPTrace = 2, kPShade1 = 3, kPShade2 = 4, kPShade3 = 5,
    kPLight = 6,    // next-event estimation: add the last light's term, set up the next shadow ray
    kPShadow = 7,   // next-event estimation: shadow-ray traversal
-   kPHold = 8,     // this iteration only: a lane that waits for its wave to gather
-   // a hit on a textured mesh: its texture record, its corner coordinates, its
-   // texel, then the faces (Shade3 with normals, Shade4 without)
-   kPTex1 = 9, kPTex2 = 10, kPTex3 = 11, kPShade4 = 12
+   kPHold = 8      // this iteration only: a lane that waits for its wave to gather
 };
 // Lane word: bits 0-3 state, bit 4 the reference's debug pixel, bits 5-10 the
 // path's chunk in the round, bits 11-30 its bounce index, bit 31 set when the
@@ -880,10 +879,10 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
    s.sp = 0;
    uint32_t mode = kPIdle;       // lane word
    uint32_t slot = 0;
-   uint32_t t0 = 0;              // Shade1 -> Shade2 -> Tex1: mesh; Shade2/Tex3 -> Shade3/4: brdf; Light/Shadow: next light
-   float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2/Tex3 -> Shade3/4: reflectance; Light/Shadow: visible, last vertex
-   // Shade1 .. Tex3 also keep the hit's face, texture record and texel address in
-   // the ray origin and reciprocal-direction words, which a found hit no longer needs
+   uint32_t t0 = 0;              // Shade1 -> Shade2: mesh; Shade2 -> Shade3: brdf; Light/Shadow: next light
+   float t1 = 0.f, t2 = 0.f, t3 = 0.f;   // Shade2 -> Shade3: reflectance; Light/Shadow: visible, last vertex
+   // Shade1 -> Shade2 of a textured hit keeps the texel's address and the texture
+   // record's flags in ray words a found hit no longer needs (origin, 1 / d)
    float src[3] = {1.f, 1.f, 1.f};
    float direct[3] = {0.f, 0.f, 0.f};    // next-event sum of the path (kNee)
    float pend[3] = {0.f, 0.f, 0.f};      // the light term a running shadow ray decides (kNee)
@@ -924,10 +923,10 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
       }
       const uint32_t m0 = mode & kPModeMask;
       if (!__any(m0 != kPIdle)) break;
-      const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3 || (kTex && m0 == kPShade4);
+      const bool heavy = m0 == kPPrim || m0 == kPShade2 || m0 == kPShade3;
       const bool tracing = m0 == kPTrace || m0 == kPShadow;
       // single-load steps that never wait for the wave
-      const bool light_step = m0 == kPShade1 || m0 == kPLight || (kTex && m0 >= kPTex1 && m0 <= kPTex3);
+      const bool light_step = m0 == kPShade1 || m0 == kPLight;
       const bool run_heavy = (uint32_t)__popcll(__ballot(heavy)) >= shade_min || !__any(tracing || light_step);
       // leaf steps gather the same way; lanes at interior nodes, in Shade1 or in
       // Light always step, so some lane progresses whenever any lane holds
@@ -951,19 +950,27 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          f.q[8] = q2.x; f.q[9] = q2.y; f.q[10] = q2.z; f.q[11] = q2.w;
          f.q[12] = q3.x; f.q[13] = q3.y; f.q[14] = q3.z; f.q[15] = q3.w;
       } else if (m == kPShade1) {
-         const PTX_GLOBAL uint32_t *pp = as_global(ssc[sch[lw_chunk(mode)].scene].pairs) + 2 * (size_t)s.hit;
-         if (kTex) {                                // (mesh, face)
-            const u32x2 pr = *reinterpret_cast<const PTX_GLOBAL u32x2 *>(pp);
-            f.q[0] = __uint_as_float(pr.x);
-            f.q[1] = __uint_as_float(pr.y);
-         } else {
-            f.q[0] = __uint_as_float(pp[0]);
+         const DevScene &S = ssc[sch[lw_chunk(mode)].scene];
+         f.q[0] = __uint_as_float(as_global(S.pairs)[2 * (size_t)s.hit]);
+         if (kTex) {                                // the triangle's texture record (flags 0 without one)
+            f.q[7] = 0.f;
+            if (S.tex_tris) {
+               const PTX_GLOBAL f32x4u *tr = as_global(reinterpret_cast<const f32x4u *>(S.tex_tris + s.hit));
+               const f32x4u a = tr[0], b = tr[1];
+               f.q[1] = a.x; f.q[2] = a.y; f.q[3] = a.z; f.q[4] = a.w;
+               f.q[5] = b.x; f.q[6] = b.y; f.q[7] = b.z; f.q[8] = b.w;
+            }
          }
       } else if (m == kPShade2) {
-         const PTX_GLOBAL uint32_t *mr =
-               as_global(reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0));
+         if (kTex && (__float_as_uint(s.r.oy) & kTexTextured)) {      // the texel instead of the mesh record
+            f.q[0] = __uint_as_float(*as_global(reinterpret_cast<const uint32_t *>(
+                  (uint64_t)__float_as_uint(s.r.ix) | ((uint64_t)__float_as_uint(s.r.iy) << 32))));
+         } else {
+            const PTX_GLOBAL uint32_t *mr =
+                  as_global(reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0));
 #pragma unroll
-         for (int k = 0; k < 9; k++) f.q[k] = __uint_as_float(mr[k]);
+            for (int k = 0; k < 9; k++) f.q[k] = __uint_as_float(mr[k]);
+         }
          const PTX_GLOBAL float *v = as_global(tris + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) f.q[9 + k] = v[k];
@@ -972,24 +979,6 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          const PTX_GLOBAL float *nrm = as_global(ssc[sch[lw_chunk(mode)].scene].normals + 9 * (size_t)s.hit);
 #pragma unroll
          for (int k = 0; k < 9; k++) { f.q[k] = v[k]; f.q[9 + k] = nrm[k]; }
-      } else if (kTex && m == kPShade4) {
-         const PTX_GLOBAL float *v = as_global(tris + 9 * (size_t)s.hit);
-#pragma unroll
-         for (int k = 0; k < 9; k++) f.q[k] = v[k];
-      } else if (kTex && m == kPTex1) {            // MeshRec words 9-13: tex_wh, tex_uv, texels
-         const PTX_GLOBAL uint32_t *mr =
-               as_global(reinterpret_cast<const uint32_t *>(ssc[sch[lw_chunk(mode)].scene].meshes + t0));
-#pragma unroll
-         for (int k = 0; k < 5; k++) f.q[k] = __uint_as_float(mr[9 + k]);
-      } else if (kTex && m == kPTex2) {            // the face's corner coordinates
-         const PTX_GLOBAL float *uv = as_global(reinterpret_cast<const float *>(
-               (uint64_t)__float_as_uint(s.r.ix) | ((uint64_t)__float_as_uint(s.r.iy) << 32)));
-#pragma unroll
-         for (int k = 0; k < 6; k++) f.q[k] = uv[k];
-      } else if (kTex && m == kPTex3) {            // the texel
-         const PTX_GLOBAL uint32_t *tx = as_global(reinterpret_cast<const uint32_t *>(
-               (uint64_t)__float_as_uint(s.r.ix) | ((uint64_t)__float_as_uint(s.r.iy) << 32)));
-         f.q[0] = __uint_as_float(*tx);
       } else if (kNee && m == kPLight) {
          const PTX_GLOBAL f32x4u *rec = as_global(reinterpret_cast<const f32x4u *>(nee_scratch + 3 * (size_t)gtid));
          const f32x4u a = rec[0], b = rec[1], c = rec[2];
@@ -1112,37 +1101,42 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          }
       } else if (m == kPShade1) {
          t0 = __float_as_uint(f.q[0]);
-         if (kTex) s.r.ox = f.q[1];                // the hit's face, for a textured mesh
+         if (kTex) {
+            const uint32_t wh = __float_as_uint(f.q[7]);
+            s.r.oy = f.q[7];                       // the texture record's flags
+            if (wh & kTexTextured) {               // the texel's address, for Shade2's load
+               const DevScene &S = ssc[sch[lw_chunk(mode)].scene];
+               const uint32_t k = __float_as_uint(f.q[8]) + texel_index(&f.q[1], s.hu, s.hv, s.hw, wh);
+               const uint64_t a = reinterpret_cast<uint64_t>(texel_pool(S) + k);
+               s.r.ix = __uint_as_float((uint32_t)a);
+               s.r.iy = __uint_as_float((uint32_t)(a >> 32));
+            }
+         }
          mode = lw_with_state(mode, kPShade2);
-      } else if (kTex && m == kPTex1) {
-         const uint32_t face = __float_as_uint(s.r.ox);
-         const uint64_t uv = ((uint64_t)__float_as_uint(f.q[1]) | ((uint64_t)__float_as_uint(f.q[2]) << 32)) +
-                             (uint64_t)face * 24u;
-         s.r.oz = f.q[0];                          // width | height << 16
-         s.r.ix = __uint_as_float((uint32_t)uv);
-         s.r.iy = __uint_as_float((uint32_t)(uv >> 32));
-         s.r.iz = f.q[3];                          // texels
-         s.r.Sx = f.q[4];
-         mode = lw_with_state(mode, kPTex2);
-      } else if (kTex && m == kPTex2) {
-         const uint32_t k = texel_index(&f.q[0], s.hu, s.hv, s.hw, __float_as_uint(s.r.oz));
-         const uint64_t tx = ((uint64_t)__float_as_uint(s.r.iz) | ((uint64_t)__float_as_uint(s.r.Sx) << 32)) +
-                             (uint64_t)k * 4u;
-         s.r.ix = __uint_as_float((uint32_t)tx);
-         s.r.iy = __uint_as_float((uint32_t)(tx >> 32));
-         mode = lw_with_state(mode, kPTex3);
-      } else if (kTex && m == kPTex3) {
-         float refl[3];
-         texel_refl(__float_as_uint(f.q[0]), refl);
-         t0 = (uint32_t)PTX_BRDF_LAMBERT;
-         t1 = refl[0]; t2 = refl[1]; t3 = refl[2];
-         const bool normals = (__float_as_uint(s.r.oy) & kRecNormals) && ssc[sch[lw_chunk(mode)].scene].normals;
-         mode = lw_with_state(mode, normals ? kPShade3 : kPShade4);
-      } else if (m == kPShade2 || m == kPShade3 || (kTex && m == kPShade4)) {
+      } else if (m == kPShade2 || m == kPShade3) {
          const LdsChunk &c = sch[lw_chunk(mode)];
          const uint32_t b = lw_bounce(mode);
          bool go = true;
-         if (m == kPShade2) {
+         if (kTex && m == kPShade2 && (__float_as_uint(s.r.oy) & kTexTextured)) {
+            // textured Lambert (never an emitter): the texel is the reflectance
+            if (b >= c.max_depth) {
+               write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
+               mode = kPIdle;
+               go = false;
+            } else {
+               float refl[3];
+               texel_refl(__float_as_uint(f.q[0]), refl);
+               t0 = (uint32_t)PTX_BRDF_LAMBERT;
+               t1 = refl[0]; t2 = refl[1]; t3 = refl[2];
+               if ((__float_as_uint(s.r.oy) & kTexNormals) && ssc[c.scene].normals) {
+                  mode = lw_with_state(mode, kPShade3);
+                  go = false;
+               } else {
+#pragma unroll
+                  for (int k = 0; k < 9; k++) f.q[k] = f.q[9 + k];
+               }
+            }
+         } else if (m == kPShade2) {
             const int32_t brdf = __float_as_int(f.q[0]);
             const int32_t is_light = __float_as_int(f.q[4]);
             const uint32_t mflags = __float_as_uint(f.q[8]);
@@ -1155,10 +1149,6 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             } else if (b >= c.max_depth) {
                write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
                mode = kPIdle;
-               go = false;
-            } else if (kTex && (mflags & kRecTextured)) {     // t0 still holds the mesh
-               s.r.oy = f.q[8];
-               mode = lw_with_state(mode, kPTex1);
                go = false;
             } else if ((mflags & kRecNormals) && ssc[c.scene].normals) {
                t0 = (uint32_t)brdf;
