@@ -20,18 +20,31 @@ int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra);
 namespace ptx {
 
 uint32_t render_lds_depth();
-int render_blocks_per_cu(int which);   // 0 extend, 1 primary, 2 plain megakernel, 3 plain with next-event estimation
+// 0 extend, 1 primary, 2 plain megakernel, 3 plain with next-event estimation, 4 shadow connection
+int render_blocks_per_cu(int which);
 void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t leaf_min, uint32_t grid,
                     hipStream_t st);
+// Shadow connection of the wavefront form (rounds with next-event jobs):
+// the vertex queue (48 B per entry, one entry per path slot at most), its
+// count, the per-slot next-event sums (3 floats) and the connect grid.  q null:
+// no next-event job in the round.
+struct ShadowArgs {
+   float4 *q = nullptr;
+   uint32_t *count = nullptr;
+   float *direct = nullptr;
+   uint32_t grid = 0;
+};
 void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs,
                         const DevScene *scenes, float *res, float *q_out, uint32_t *count_out, uint32_t *bounce,
-                        uint32_t grid, hipStream_t st);
+                        const ShadowArgs &sh, uint4 *spill, uint32_t max_stack, uint32_t grid, hipStream_t st);
 void launch_bounce(const float *q_in, const uint32_t *count_in, float *q_out, uint32_t *count_out, float *hits,
                    const JobDev *jobs, const DevScene *scenes, float *res, uint32_t *work, uint4 *spill,
-                   uint32_t max_stack, uint32_t leaf_min, uint32_t *bounce, uint32_t ext_grid, uint32_t shade_grid,
-                   hipStream_t st);
-void launch_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs, const float *res, uint32_t grid,
-                       hipStream_t st);
+                   uint32_t max_stack, uint32_t leaf_min, uint32_t *bounce, const ShadowArgs &sh, uint32_t ext_grid,
+                   uint32_t shade_grid, hipStream_t st);
+// direct: the per-slot next-event sums of a wavefront round, added to the
+// samples of next-event jobs; null for path-regeneration rounds.
+void launch_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs, const float *res,
+                       const float *direct, uint32_t grid, hipStream_t st);
 void launch_finalize(float *film, uint32_t npix, uint32_t spp, hipStream_t st);
 void launch_zero_floats(float *a, size_t n, hipStream_t st);
 int paths_blocks_per_cu();

@@ -24,6 +24,7 @@
 // identical to each other and to the CPU restatement.
 #include <hip/hip_runtime.h>
 
+#include "launch.h"
 #include "traverse.h"
 
 #pragma clang fp contract(off)
@@ -449,6 +450,31 @@ __device__ __forceinline__ void enqueue(const QueueRefs &Q, bool want, const flo
    }
 }
 
+// Shadow-connect queue of the wavefront form (next-event estimation): one
+// record per Lambert vertex of a next-event job, 12 words: the vertex P and the
+// path slot, the shading normal and the scene, throughput x Kd.
+struct ShadowRefs {
+   float4 *q;                  // [capacity][3], or null: no next-event job in the round
+   uint32_t *count;
+};
+
+__device__ __forceinline__ void enqueue_shadow(const ShadowRefs &Q, bool want, const Frame &fr, uint32_t slot,
+                                               uint32_t scene, const float k[3]) {
+   const uint64_t m = __ballot(want);
+   if (!m) return;
+   const uint32_t lane = __lane_id();
+   const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+   uint32_t base = 0;
+   if (lane == leader) base = atomicAdd(Q.count, (uint32_t)__popcll(m));
+   base = __shfl(base, leader);
+   if (want) {
+      float4 *dst = Q.q + 3 * (size_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
+      dst[0] = make_float4(fr.lx, fr.ly, fr.lz, __uint_as_float(slot));
+      dst[1] = make_float4(fr.nx, fr.ny, fr.nz, __uint_as_float(scene));
+      dst[2] = make_float4(k[0], k[1], k[2], 0.f);
+   }
+}
+
 __device__ __forceinline__ void write_res(float *res, uint32_t path, Radiance L) {
    res[3 * (size_t)path] = L.r;
    res[3 * (size_t)path + 1] = L.g;
@@ -467,11 +493,19 @@ __device__ __forceinline__ uint32_t find_chunk(const Chunk *chunks, uint32_t nch
 // Shades every path slot of a round at its primary hit (bounce 0).
 __global__ void __launch_bounds__(256) k_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots,
                                                      const JobDev *jobs, const DevScene *scenes, float *res,
-                                                     QueueRefs out) {
+                                                     QueueRefs out, ShadowRefs shadow, float *direct) {
    for (uint32_t slot0 = blockIdx.x * blockDim.x; slot0 < nslots; slot0 += gridDim.x * blockDim.x) {
       const uint32_t slot = slot0 + threadIdx.x;
-      bool want = false;
+      bool want = false, want_shadow = false;
       float e[kQWords];
+      Frame fr;
+      float kd[3] = {0.f, 0.f, 0.f};
+      uint32_t scene = 0;
+      if (slot < nslots && shadow.q) {
+         direct[3 * (size_t)slot] = 0.f;
+         direct[3 * (size_t)slot + 1] = 0.f;
+         direct[3 * (size_t)slot + 2] = 0.f;
+      }
       if (slot < nslots) {
          const Chunk c = chunks[find_chunk(chunks, nchunks, slot)];
          const uint32_t local = slot - c.base;
@@ -491,13 +525,17 @@ __global__ void __launch_bounds__(256) k_round_start(const Chunk *chunks, uint32
             } else {
                M = S.meshes[w0 >> 2];
             }
-            Frame fr;
             fr.lx = a.y; fr.ly = a.z; fr.lz = a.w;
             fr.t1x = b.x; fr.t1y = b.y; fr.t1z = b.z;
             fr.nx = b.w; fr.ny = cc.x; fr.nz = cc.y;
             fr.t2x = cc.z; fr.t2y = cc.w; fr.t2z = d.x;
             const float lin[3] = {d.y, d.z, d.w};
             float src[3] = {1.f, 1.f, 1.f};
+            if (shadow.q && (J.flags & PTX_JOB_NEE) && M.brdf == PTX_BRDF_LAMBERT && S.num_lights > 0) {
+               want_shadow = true;
+               kd[0] = src[0] * M.refl[0]; kd[1] = src[1] * M.refl[1]; kd[2] = src[2] * M.refl[2];
+               scene = J.scene;
+            }
             float wout[3];
             bounce(M, fr, lin, pixel, sample, 0u, J.seed, wout, src);
             if (src_is_zero(src)) {
@@ -516,6 +554,7 @@ __global__ void __launch_bounds__(256) k_round_start(const Chunk *chunks, uint32
          }
       }
       enqueue(out, want, e);
+      if (shadow.q) enqueue_shadow(shadow, want_shadow, fr, slot, scene, kd);
    }
 }
 
@@ -583,21 +622,25 @@ k_extend(const float *queue, const uint32_t *count, const DevScene *scenes, floa
 // Shade stage at bounce b >= 1 (PathTraceIntegrator.cpp:23-95).
 __global__ void __launch_bounds__(256) k_shade(const float *queue_in, const uint32_t *count_in, const float *hits,
                                                const JobDev *jobs, const DevScene *scenes, float *res,
-                                               QueueRefs out, const uint32_t *bounce_ptr) {
+                                               QueueRefs out, const uint32_t *bounce_ptr, ShadowRefs shadow) {
    const uint32_t n = *count_in;
    const uint32_t b = *bounce_ptr;
    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
       const uint32_t i = i0 + threadIdx.x;
-      bool want = false;
+      bool want = false, want_shadow = false;
       float e[kQWords];
+      Frame fr;
+      float kd[3] = {0.f, 0.f, 0.f};
+      uint32_t path = 0, scene = 0;
       if (i < n) {
          const float4 *qe = reinterpret_cast<const float4 *>(queue_in + (size_t)kQWords * i);
          const float4 a = qe[0], bq = qe[1], c = qe[2], d = qe[3];
-         const uint32_t path = __float_as_uint(a.x), job = __float_as_uint(a.y);
+         path = __float_as_uint(a.x);
+         const uint32_t job = __float_as_uint(a.y);
          const uint32_t pixel = __float_as_uint(a.z), sample = __float_as_uint(a.w);
          const float dir[3] = {bq.w, c.x, c.y};
          float src[3] = {c.z, c.w, d.x};
-         const uint32_t scene = __float_as_uint(d.z);
+         scene = __float_as_uint(d.z);
          const float4 *hp = reinterpret_cast<const float4 *>(hits + 8 * (size_t)path);
          const float4 h0 = hp[0], h1 = hp[1];
          const uint32_t tri = __float_as_uint(h0.x);
@@ -616,7 +659,11 @@ __global__ void __launch_bounds__(256) k_shade(const float *queue_in, const uint
             } else {
                MeshRec M = Mg;
                if (M.flags & kRecTextured) texel_refl(tri_texel(S, tri, h0.z, h0.w, h1.x), M.refl);
-               const Frame fr = frame_of(S, tri, dir, h0.z, h0.w, h1.x, __float_as_uint(h1.y), mesh);
+               fr = frame_of(S, tri, dir, h0.z, h0.w, h1.x, __float_as_uint(h1.y), mesh);
+               if (shadow.q && (J.flags & PTX_JOB_NEE) && M.brdf == PTX_BRDF_LAMBERT && S.num_lights > 0) {
+                  want_shadow = true;
+                  kd[0] = src[0] * M.refl[0]; kd[1] = src[1] * M.refl[1]; kd[2] = src[2] * M.refl[2];
+               }
                float lin[3], wout[3];
                world_to_local(fr, dir, lin);
                bounce(M, fr, lin, pixel, sample, b, J.seed, wout, src);
@@ -634,22 +681,64 @@ __global__ void __launch_bounds__(256) k_shade(const float *queue_in, const uint
          }
       }
       enqueue(out, want, e);
+      if (shadow.q) enqueue_shadow(shadow, want_shadow, fr, path, scene, kd);
+   }
+}
+
+// Shadow-connect stage of the wavefront form: for each queued vertex, the
+// point lights in order (light_setup, a bounded shadow traversal), adding
+// (throughput x Kd) x (I x cos / (pi d^2)) of each unoccluded light to the
+// path's next-event sum.  One vertex per thread; a path has at most one
+// vertex per bounce, so its sum takes the terms in the order the oracle does.
+template <int kLds>
+__global__ void __launch_bounds__(256) k_connect(const float4 *queue, const uint32_t *count, const DevScene *scenes,
+                                                 float *direct, uint4 *spill, uint32_t max_stack) {
+   __shared__ uint32_t lds_ref[kLds * 256];
+   __shared__ float lds_near[kLds * 256];
+   __shared__ uint16_t lds_cnt[kLds * 256];
+   const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
+                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, max_stack};
+   const uint32_t n = *count;
+   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      const float4 a = queue[3 * (size_t)i], b = queue[3 * (size_t)i + 1], c = queue[3 * (size_t)i + 2];
+      const uint32_t slot = __float_as_uint(a.w);
+      const DevScene &S = scenes[__float_as_uint(b.w)];
+      float *dsum = direct + 3 * (size_t)slot;
+      float dr = dsum[0], dg = dsum[1], db = dsum[2];
+      for (uint32_t j = 0; j < S.num_lights; j++) {
+         const float *light = S.lights + 6 * (size_t)j;
+         float l[3], dist, g;
+         if (!light_setup(light, a.x, a.y, a.z, b.x, b.y, b.z, l, dist, g)) continue;
+         if (occluded<kLds>(S, st, a.x, a.y, a.z, l, dist)) continue;
+         dr = dr + c.x * (light[3] * g);
+         dg = dg + c.y * (light[4] * g);
+         db = db + c.z * (light[5] * g);
+      }
+      dsum[0] = dr; dsum[1] = dg; dsum[2] = db;
    }
 }
 
 // Adds each chunk's samples to its job's film, per pixel in sample order, each
 // sample clamped to 255 per channel (BoxFilter::AddSample/Output).
 __global__ void __launch_bounds__(256) k_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs,
-                                                    const float *res) {
+                                                    const float *res, const float *direct) {
    for (uint32_t ci = 0; ci < nchunks; ci++) {
       const Chunk c = chunks[ci];
       const JobDev &J = jobs[c.job];
+      // wavefront form: a next-event job's sample is its terminal radiance plus its next-event sum
+      const bool add = direct && (J.flags & PTX_JOB_NEE);
       for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < c.npix; p += gridDim.x * blockDim.x) {
          float *film = J.film + 3 * (size_t)p;
          float ar = film[0], ag = film[1], ab = film[2];
          for (uint32_t s = 0; s < c.k; s++) {
-            const float *r = res + 3 * ((size_t)c.base + (size_t)s * c.npix + p);
+            const size_t slot = (size_t)c.base + (size_t)s * c.npix + p;
+            const float *r = res + 3 * slot;
             float sr = r[0], sg = r[1], sb = r[2];
+            if (add) {
+               sr = sr + direct[3 * slot];
+               sg = sg + direct[3 * slot + 1];
+               sb = sb + direct[3 * slot + 2];
+            }
             if (sr > 255) sr = 255;
             if (sg > 255) sg = 255;
             if (sb > 255) sb = 255;
@@ -671,11 +760,12 @@ __global__ void k_finalize(float *film, uint32_t npix, uint32_t spp) {
 
 // Start of a bounce: clears the traversal work counters and the next queue's
 // count, and advances the device-resident bounce index.
-__global__ void k_bounce_prologue(uint32_t *work, uint32_t *count_out, uint32_t *bounce) {
+__global__ void k_bounce_prologue(uint32_t *work, uint32_t *count_out, uint32_t *bounce, uint32_t *count_shadow) {
    const uint32_t i = threadIdx.x;
    if (i < 8) work[i] = 0u;
    if (i == 8) *count_out = 0u;
    if (i == 9) *bounce += 1u;
+   if (i == 10 && count_shadow) *count_shadow = 0u;
 }
 
 __global__ void k_zero_words(uint32_t *a, uint32_t na, uint32_t *b, uint32_t nb) {
@@ -1212,7 +1302,8 @@ int render_blocks_per_cu(int which) {
    if (which == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_extend<kRenderLds>, 256, 0);
    else if (which == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_primary<kRenderLds>, 256, 0);
    else if (which == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, false, false>, 256, 0);
-   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, true, false>, 256, 0);
+   else if (which == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, true, false>, 256, 0);
+   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_connect<kRenderLds>, 256, 0);
    return e == hipSuccess && b > 0 ? b : 1;
 }
 
@@ -1224,28 +1315,35 @@ void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *s
 
 void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs,
                         const DevScene *scenes, float *res, float *q_out, uint32_t *count_out, uint32_t *bounce,
-                        uint32_t grid, hipStream_t st) {
-   hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, count_out, 1u, bounce, 1u);
+                        const ShadowArgs &sh, uint4 *spill, uint32_t max_stack, uint32_t grid, hipStream_t st) {
+   // the bounce index and the shadow count are adjacent words
+   hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, count_out, 1u, bounce, sh.q ? 2u : 1u);
    hipLaunchKernelGGL(dev::k_round_start, dim3(grid), dim3(256), 0, st, chunks, nchunks, nslots, jobs, scenes, res,
-                      dev::QueueRefs{q_out, count_out});
+                      dev::QueueRefs{q_out, count_out}, dev::ShadowRefs{sh.q, sh.count}, sh.direct);
+   if (sh.q)
+      hipLaunchKernelGGL(dev::k_connect<kRenderLds>, dim3(sh.grid), dim3(256), 0, st, sh.q, sh.count, scenes,
+                         sh.direct, spill, max_stack);
 }
 
-// One bounce: prologue, extend, shade.  Captured once per queue direction and
-// replayed for every bounce.
+// One bounce: prologue, extend, shade, and with next-event jobs the shadow
+// connection.  Captured once per queue direction and replayed for every bounce.
 void launch_bounce(const float *q_in, const uint32_t *count_in, float *q_out, uint32_t *count_out, float *hits,
                    const JobDev *jobs, const DevScene *scenes, float *res, uint32_t *work, uint4 *spill,
-                   uint32_t max_stack, uint32_t leaf_min, uint32_t *bounce, uint32_t ext_grid, uint32_t shade_grid,
-                   hipStream_t st) {
-   hipLaunchKernelGGL(dev::k_bounce_prologue, dim3(1), dim3(64), 0, st, work, count_out, bounce);
+                   uint32_t max_stack, uint32_t leaf_min, uint32_t *bounce, const ShadowArgs &sh, uint32_t ext_grid,
+                   uint32_t shade_grid, hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_bounce_prologue, dim3(1), dim3(64), 0, st, work, count_out, bounce, sh.count);
    hipLaunchKernelGGL(dev::k_extend<kRenderLds>, dim3(ext_grid), dim3(256), 0, st, q_in, count_in, scenes, hits, work,
                       spill, max_stack, leaf_min);
    hipLaunchKernelGGL(dev::k_shade, dim3(shade_grid), dim3(256), 0, st, q_in, count_in, hits, jobs, scenes, res,
-                      dev::QueueRefs{q_out, count_out}, bounce);
+                      dev::QueueRefs{q_out, count_out}, bounce, dev::ShadowRefs{sh.q, sh.count});
+   if (sh.q)
+      hipLaunchKernelGGL(dev::k_connect<kRenderLds>, dim3(sh.grid), dim3(256), 0, st, sh.q, sh.count, scenes,
+                         sh.direct, spill, max_stack);
 }
 
-void launch_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs, const float *res, uint32_t grid,
-                       hipStream_t st) {
-   hipLaunchKernelGGL(dev::k_accumulate, dim3(grid), dim3(256), 0, st, chunks, nchunks, jobs, res);
+void launch_accumulate(const Chunk *chunks, uint32_t nchunks, const JobDev *jobs, const float *res,
+                       const float *direct, uint32_t grid, hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_accumulate, dim3(grid), dim3(256), 0, st, chunks, nchunks, jobs, res, direct);
 }
 
 void launch_finalize(float *film, uint32_t npix, uint32_t spp, hipStream_t st) {

@@ -79,10 +79,12 @@ size_t job_device_bytes(const ptx_job_desc &j) {
 RenderEngine::~RenderEngine() { release(); }
 
 void RenderEngine::release() {
-   if (g_ab) (void)hipGraphExecDestroy(g_ab);
-   if (g_ba) (void)hipGraphExecDestroy(g_ba);
-   g_ab = g_ba = nullptr;
-   dfree(qa); dfree(qb); dfree(hits); dfree(res); dfree(words); dfree(spill); dfree(nee_scratch);
+   for (hipGraphExec_t *g : {&g_ab, &g_ba, &g_ab_nee, &g_ba_nee}) {
+      if (*g) (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+   }
+   dfree(qa); dfree(qb); dfree(hits); dfree(qs); dfree(direct); dfree(res); dfree(words); dfree(spill);
+   dfree(nee_scratch);
    dfree(d_jobs); dfree(d_scenes); dfree(d_chunks);
    cap = qcap = 0;
    spill_entries = 0;
@@ -90,7 +92,8 @@ void RenderEngine::release() {
 }
 
 size_t RenderEngine::bytes_for(uint32_t paths, bool queues) {
-   return (size_t)paths * (3 * sizeof(float) + (queues ? 2 * 16 * sizeof(float) + 8 * sizeof(float) : 0));
+   // results; queues: two path queues, hit records, the shadow queue and the next-event sums
+   return (size_t)paths * (3 * sizeof(float) + (queues ? (2 * 16 + 8 + 12 + 3) * sizeof(float) : 0));
 }
 
 bool RenderEngine::regen_round(uint32_t nchunks, uint32_t nscenes) const {
@@ -106,8 +109,9 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       mega_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(2));
       mega_nee_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(3));
       paths_grid = (uint32_t)(dev->num_cus * paths_blocks_per_cu());
+      connect_grid = (uint32_t)(dev->num_cus * render_blocks_per_cu(4));
    }
-   const uint32_t stride_threads = std::max({ext_grid, mega_grid, mega_nee_grid, paths_grid}) * 256;
+   const uint32_t stride_threads = std::max({ext_grid, mega_grid, mega_nee_grid, paths_grid, connect_grid}) * 256;
    const size_t need_spill = max_stack > lds ? (size_t)(max_stack - lds) * stride_threads : 0;
    int rc;
    if (paths > cap) {
@@ -156,11 +160,12 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
 int RenderEngine::ensure_queues(uint32_t need) {
    int rc;
    if (qcap < need) {
-      dfree(qa); dfree(qb); dfree(hits);
+      dfree(qa); dfree(qb); dfree(hits); dfree(qs); dfree(direct);
       qcap = 0;
       if ((rc = dalloc(&qa, (size_t)need * 16)) || (rc = dalloc(&qb, (size_t)need * 16)) ||
-          (rc = dalloc(&hits, (size_t)need * 8))) {
-         dfree(qa); dfree(qb); dfree(hits);
+          (rc = dalloc(&hits, (size_t)need * 8)) || (rc = dalloc(&qs, (size_t)need * 3)) ||
+          (rc = dalloc(&direct, (size_t)need * 3))) {
+         dfree(qa); dfree(qb); dfree(hits); dfree(qs); dfree(direct);
          return rc;
       }
       qcap = need;
@@ -171,22 +176,26 @@ int RenderEngine::ensure_queues(uint32_t need) {
       graphs_stale = true;
    }
    if (graphs_stale || !g_ab) {
-      if (g_ab) (void)hipGraphExecDestroy(g_ab);
-      if (g_ba) (void)hipGraphExecDestroy(g_ba);
-      g_ab = g_ba = nullptr;
-      for (int dir = 0; dir < 2; dir++) {
+      for (hipGraphExec_t *g : {&g_ab, &g_ba, &g_ab_nee, &g_ba_nee}) {
+         if (*g) (void)hipGraphExecDestroy(*g);
+         *g = nullptr;
+      }
+      for (int variant = 0; variant < 4; variant++) {
+         const int dir = variant & 1;
+         const bool nee = variant >= 2;
          hipGraph_t g = nullptr;
          PTX_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
          float *qin = dir == 0 ? qa : qb, *qout = dir == 0 ? qb : qa;
          uint32_t *cin = words + (dir == 0 ? kCountA : kCountB), *cout = words + (dir == 0 ? kCountB : kCountA);
          launch_bounce(qin, cin, qout, cout, hits, d_jobs, d_scenes, res, words + kWork, spill, stack_cap,
-                       leaf_lanes_of(dev, true), words + kBounce, ext_grid, shade_grid, stream);
+                       leaf_lanes_of(dev, true), words + kBounce, nee ? shadow_args() : ShadowArgs{}, ext_grid,
+                       shade_grid, stream);
          PTX_HIP_TRY(hipStreamEndCapture(stream, &g));
          hipGraphExec_t ex = nullptr;
          const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
          (void)hipGraphDestroy(g);
          if (ie != hipSuccess) return set_error(PTX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-         (dir == 0 ? g_ab : g_ba) = ex;
+         (nee ? (dir == 0 ? g_ab_nee : g_ba_nee) : (dir == 0 ? g_ab : g_ba)) = ex;
       }
       graphs_stale = false;
    }
@@ -244,9 +253,7 @@ int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint3
                        bool nee, bool glossy, bool textured) {
    uint32_t nslots = 0;
    for (const Chunk &c : r) nslots = std::max(nslots, c.base + c.k * c.npix);
-   if (nee && !regen_round((uint32_t)r.size(), nscenes))
-      return set_error(PTX_ERR_INVALID, "next-event estimation renders in the path-regeneration form only");
-   if (nee && !nee_scratch) {
+   if (nee && regen_round((uint32_t)r.size(), nscenes) && !nee_scratch) {
       const int rc = dalloc(&nee_scratch, (size_t)paths_grid * 256 * 3);
       if (rc) return rc;
    }
@@ -255,15 +262,18 @@ int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint3
       launch_paths(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, nscenes, res, words + kWork, spill, stack_cap,
                    shade_min, leaf_lanes_of(dev, true), nee ? nee_scratch : nullptr, glossy, textured,
                    std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
-      launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
+      launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, nullptr, shade_grid, stream);
       return PTX_OK;
    }
    const int rc = ensure_queues(nslots);
    if (rc) return rc;
+   const ShadowArgs sh = nee ? shadow_args() : ShadowArgs{};
    launch_round_start(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, res, qa, words + kCountA,
-                      words + kBounce, std::min<uint32_t>(shade_grid, (nslots + 255) / 256), stream);
-   for (uint32_t b = 1; b <= depth; b++) (void)hipGraphLaunch(b & 1 ? g_ab : g_ba, stream);
-   launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, shade_grid, stream);
+                      words + kBounce, sh, spill, stack_cap, std::min<uint32_t>(shade_grid, (nslots + 255) / 256),
+                      stream);
+   for (uint32_t b = 1; b <= depth; b++)
+      (void)hipGraphLaunch(b & 1 ? (nee ? g_ab_nee : g_ab) : (nee ? g_ba_nee : g_ba), stream);
+   launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, nee ? direct : nullptr, shade_grid, stream);
    return PTX_OK;
 }
 

@@ -23,11 +23,12 @@ std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jo
                                                 uint32_t max_chunks);
 
 struct RenderEngine {
-   static constexpr uint32_t kDefaultPaths = 1u << 22;        // wavefront rounds (172 B per slot)
+   static constexpr uint32_t kDefaultPaths = 1u << 22;        // wavefront rounds (232 B per slot)
    static constexpr uint32_t kDefaultRegenPaths = 1u << 28;   // regeneration rounds (12 B per slot)
    static constexpr uint32_t kDefaultShadeLanes = 16;   // lanes a wave gathers before it shades
-   // layout of `words`: 8 traversal work counters, the two queue counts, the bounce index
-   static constexpr uint32_t kWork = 0, kCountA = 8, kCountB = 9, kBounce = 10;
+   // layout of `words`: 8 traversal work counters, the two queue counts, the
+   // bounce index, the shadow-queue count
+   static constexpr uint32_t kWork = 0, kCountA = 8, kCountB = 9, kBounce = 10, kCountS = 11;
 
    ptx_device *dev = nullptr;
    hipStream_t stream = nullptr;
@@ -35,6 +36,8 @@ struct RenderEngine {
    uint32_t qcap = 0;                       // path slots of the wavefront queues (0 = none)
    float *qa = nullptr, *qb = nullptr;      // queues, 16 words per entry
    float *hits = nullptr;                   // 8 words per path slot
+   float4 *qs = nullptr;                    // shadow-connect queue, 3 float4 per entry
+   float *direct = nullptr;                 // next-event sums of the wavefront form, 3 floats per slot
    float *res = nullptr;                    // 3 floats per path slot
    uint32_t *words = nullptr;
    uint4 *spill = nullptr;
@@ -47,8 +50,9 @@ struct RenderEngine {
    uint32_t scenes_cap = 0;
    Chunk *d_chunks = nullptr;
    uint32_t chunks_cap = 0;
-   uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0, mega_nee_grid = 0, paths_grid = 0;
-   hipGraphExec_t g_ab = nullptr, g_ba = nullptr;
+   uint32_t ext_grid = 0, shade_grid = 0, mega_grid = 0, mega_nee_grid = 0, paths_grid = 0, connect_grid = 0;
+   hipGraphExec_t g_ab = nullptr, g_ba = nullptr;        // bounce graphs per queue direction
+   hipGraphExec_t g_ab_nee = nullptr, g_ba_nee = nullptr;  // the same with the shadow connection
    bool graphs_stale = true;
    uint32_t graph_leaf = 0;                 // leaf_min the graphs were captured with
 
@@ -64,12 +68,13 @@ struct RenderEngine {
    // Wavefront queues and hit records for at least `need` slots, and the
    // per-bounce graphs.
    int ensure_queues(uint32_t need);
+   ShadowArgs shadow_args() const { return ShadowArgs{qs, words + kCountS, direct, connect_grid}; }
    // Allocates the job's film and primary records, zeroes the film, traces the
    // primary hits.  J.film / J.primary are set.
    int start_job(JobDev &J, const DevScene &S, uint32_t npix);
    // One round over the chunks in d_round (a device copy of r); the job and
    // scene tables must hold the round's jobs, the scene table nscenes entries.
-   // nee: some job of the round uses next-event estimation (regeneration only);
+   // nee: some job of the round uses next-event estimation;
    // glossy / textured: some scene of the round has glossy / textured meshes.
    int round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes, bool nee,
              bool glossy, bool textured);

@@ -164,6 +164,8 @@ def test_next_event_estimation_matches_oracle(dev, depth):
     assert np.array_equal(plain.view(np.uint32), ref.view(np.uint32))
     (batched,) = batch(dev, [(sc, cam, 3, depth, 21, True)], _abi.RENDER_REGEN)
     assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
+    (wave,) = batch(dev, [(sc, cam, 3, depth, 21, True)], _abi.RENDER_WAVEFRONT)   # shadow-connect stage
+    assert np.array_equal(wave.view(np.uint32), ref.view(np.uint32))
     if depth:
         off = O.OracleScene(meshes, sky, lights=lights).render(cam, 3, depth, seed=21)
         assert (ref >= off).all() and (ref > off).any()
@@ -180,13 +182,10 @@ def test_next_event_jobs_mixed_with_plain_jobs(dev):
     assert np.array_equal(mixed[1].view(np.uint32), alone.view(np.uint32))
     ref = O.OracleScene(meshes, sky, lights=lights).render(cam, 2, 4, seed=5, nee=True)
     assert np.array_equal(mixed[0].view(np.uint32), ref.view(np.uint32))
-    # the wavefront form has no shadow stage: such jobs are refused, not rendered wrong
-    dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT)
-    try:
-        with pytest.raises(Exception):
-            render_batch([(sc, cam, 1, 2, 5, True)])
-    finally:
-        dev.set_tuning()
+    # the wavefront form: the shadow-connect stage serves the next-event job only
+    wave = batch(dev, [(sc, cam, 2, 4, 5, True), (sc, small, 3, 4, 6)], _abi.RENDER_WAVEFRONT)
+    assert np.array_equal(wave[0].view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(wave[1].view(np.uint32), alone.view(np.uint32))
 
 
 def test_scene_and_server_outlive_their_device_handle():
