@@ -7,7 +7,9 @@ the compact BVH (32-byte nodes), the leaf (mesh, face) pairs and the 100-byte
 per-ray hit records of bvh::intersect_compact + Mesh::intersect (hit flag,
 mesh, face, t, u, v, w, Location, normal, tangents, error bound and the
 per-ray counters).  Render fixtures hold the reference film of a
-deterministic (mirror + emitter + skybox) scene.
+deterministic (mirror + emitter + skybox) scene.  The texture fixture holds
+camera rays of the textured scene that tests/test_textures.py writes and the
+reference's Scene::intersect + LambertBRDF::sample texture lookups for them.
 
 Run in a container where /root/reference exists:  python tests/golden/make_golden.py
 """
@@ -112,6 +114,25 @@ def main():
               S.MeshData(gv, gf, brdf=S.MIRROR, refl=(0.5, 0.6, 0.7)),
               S.MeshData(lv, lf, is_light=True, emission=(255.0, 240.0, 200.0))]
     save("mirror_multimesh", meshes, None, sky=(40.0, 60.0, 90.0), camera=cam, spp=2, max_depth=5)
+    # 7. textured scene file (PNG textures, PLY u/v, coordinates past the image):
+    #    the reference's texture lookups on one camera ray per pixel
+    save_tex_probe("tex_probe_crafted")
+
+
+def save_tex_probe(name):
+    from polytope_amd import pbrt as P
+    from tests.test_textures import TEX_PROBE, TEXTURED, _camera_rays, _crafted
+    with tempfile.TemporaryDirectory() as d:
+        d = Path(d)
+        _crafted(d)
+        (d / "s.pbrt").write_text(TEXTURED)
+        sf = P.read_pbrt(str(d / "s.pbrt"))
+        rays = _camera_rays(sf.camera, 1)
+        S.write_rays(str(d / "rays.bin"), rays)
+        O.run_reference(["--pbrt", str(d / "s.pbrt"), "--rays", "rays.bin", "--tex-probe", "probe.bin"], cwd=str(d))
+        probe = np.fromfile(d / "probe.bin", TEX_PROBE)
+    np.savez_compressed(OUT / f"{name}.npz", rays=rays, probe=probe.view(np.uint32).reshape(len(rays), -1))
+    print(f"wrote {name}.npz: {len(rays)} rays, {int((probe['flags'] & 8).astype(bool).sum())} texel lookups")
 
 if __name__ == "__main__":
     main()

@@ -166,6 +166,8 @@ def test_next_event_estimation_matches_oracle(dev, depth):
     assert np.array_equal(batched.view(np.uint32), ref.view(np.uint32))
     (wave,) = batch(dev, [(sc, cam, 3, depth, 21, True)], _abi.RENDER_WAVEFRONT)   # shadow-connect stage
     assert np.array_equal(wave.view(np.uint32), ref.view(np.uint32))
+    (wave_rounds,) = batch(dev, [(sc, cam, 3, depth, 21, True)], _abi.RENDER_WAVEFRONT, paths_per_wave=4096)
+    assert np.array_equal(wave_rounds.view(np.uint32), ref.view(np.uint32))          # one sample per round
     if depth:
         off = O.OracleScene(meshes, sky, lights=lights).render(cam, 3, depth, seed=21)
         assert (ref >= off).all() and (ref > off).any()

@@ -461,6 +461,36 @@ def _probe_matches(path, tmp_path, step):
     return seen
 
 
+def test_texture_lookups_match_reference_golden(tmp_path):
+    """The reference's lookups on the crafted scene (tests/golden/tex_probe_crafted.npz,
+    written by tests/golden/make_golden.py) against the oracle's, without the
+    reference build: the scene's files are regenerated here from the seeds."""
+    from pathlib import Path
+    with np.load(Path(__file__).parent / "golden" / "tex_probe_crafted.npz", allow_pickle=False) as z:
+        rays, ref = z["rays"], z["probe"].reshape(-1).view(TEX_PROBE)
+    _crafted(tmp_path)
+    (tmp_path / "s.pbrt").write_text(TEXTURED)
+    sf = P.read_pbrt(str(tmp_path / "s.pbrt"))
+    assert np.array_equal(rays, _camera_rays(sf.camera, 1))
+    hits = O.OracleScene(sf.meshes, sf.skybox).intersect(rays)
+    assert np.array_equal(ref["flags"] & 1, hits["hit"])
+    inside = 0
+    for i in np.nonzero(ref["flags"] & 4)[0]:
+        h, m = hits[i], sf.meshes[int(ref["mesh"][i])]
+        assert (h["mesh"], h["face"]) == (ref["mesh"][i], ref["face"][i])
+        f = m.faces[h["face"]]
+        tu, tv = O.tex_lerp(np.concatenate([m.uv[f[0]], m.uv[f[1]], m.uv[f[2]]]), float(h["u"]), float(h["v"]),
+                            float(h["w"]))
+        assert _bits([tu, tv]) == _bits([ref["tu"][i], ref["tv"][i]]), i
+        if ref["flags"][i] & 8:
+            k = O.texel_index(tu, tv, m.texture.width, m.texture.height)
+            assert k == ref["index"][i], i
+            px = m.texture.rgba.reshape(-1, 4)[k, :3].astype(np.float32) * np.float32(1.0 / 255.0)
+            assert _bits(px) == _bits([ref["r"][i], ref["g"][i], ref["b"][i]]), i
+            inside += 1
+    assert inside > 1000
+
+
 @needs_ref
 def test_texture_lookups_match_reference_on_crafted_scene(tmp_path):
     _crafted(tmp_path)
