@@ -158,7 +158,8 @@ def render_section(dev, args):
     same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) and
                np.array_equal(a.view(np.uint32), w.view(np.uint32)) for a, b, w in zip(films_b, films_p, films_w))
     # the same geometry and BVH with textured Lambert and glossy meshes
-    mat_scene = Scene(dev, S.with_materials(meshes), sky, bvh=scene.bvh())
+    bvh = scene.bvh()
+    mat_scene = Scene(dev, S.with_materials(meshes), sky, bvh=bvh)
     scene.close()
     mat_batch = [(mat_scene, c, spp, d, seed) for c, spp, d, seed in jobs]
     render_batch([(mat_scene, jobs[0][0], 1, 8, 1)])
@@ -172,6 +173,25 @@ def render_section(dev, args):
     print(f"[bench] textured + glossy: batched {samples / t_mat / 1e6:.1f}, plain {samples / t_mat_plain / 1e6:.1f} "
           f"Msamples/s", file=sys.stderr, flush=True)
     mat_scene.close()
+    cpu = None
+    if not args.skip_cpu:
+        # CPU baseline: the oracle (port of the reference's integrator) on the same
+        # scene and tree, the whole first job, on the host's cores
+        import oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cam0, spp0, d0, seed0 = jobs[0]
+        orc = O.OracleScene(meshes, sky, bvh=bvh)
+        del bvh
+        tc = time.perf_counter()
+        film0 = orc.render(cam0, spp0, d0, seed0, threads=threads)
+        t_cpu = time.perf_counter() - tc
+        n_cpu = cam0.height * cam0.width * spp0
+        del orc
+        cpu = {"value": round(n_cpu / t_cpu, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+               "sample": f"job 0 ({cam0.width}x{cam0.height}, {spp0} spp, depth {d0}): {n_cpu} samples in "
+                         f"{t_cpu:.2f} s, oracle restatement of PathTraceIntegrator on the same tree",
+               "film_equals_gpu": bool(np.array_equal(film0.view(np.uint32), films_b[0].view(np.uint32)))}
+        print(f"[bench] render CPU baseline: {n_cpu / t_cpu / 1e6:.3f} Msamples/s", file=sys.stderr, flush=True)
     return {"workload": f"icosphere subdiv {args.render_subdiv} x {args.render_instances + 1} instances + ground "
                         f"({sum(m.num_faces for m in meshes)} tris), 16 jobs 256x256..1920x1080, "
                         f"{args.render_spp} spp, depth 8, 70% Lambert / 30% mirror",
@@ -183,7 +203,8 @@ def render_section(dev, args):
             "materials": {"workload": "the same meshes and BVH, Lambert meshes textured (512x512 RGBA8), mirrors "
                                       "glossy (roughness 0.2)",
                           "batched_samples_per_s": round(samples / t_mat, 1),
-                          "plain_samples_per_s": round(samples / t_mat_plain, 1), "films_identical": same_m}}
+                          "plain_samples_per_s": round(samples / t_mat_plain, 1), "films_identical": same_m},
+            "cpu_baseline": cpu}
 
 
 def plain_cpu_render_section(dev, args, threads: int):
