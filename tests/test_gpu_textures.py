@@ -114,3 +114,26 @@ def test_textured_scene_rejects_bad_descriptions(dev):
     bad[0].brdf = S.MIRROR                                # only Lambert meshes take a texture
     with pytest.raises(Exception, match="Lambert"):
         Scene(dev, bad, sky)
+
+
+def test_texture_size_limits_and_far_coordinates(dev):
+    """Textures at the size limits (1x1, 32767x1, 1x32767) on quads whose
+    coordinates run far outside [0, 1] and below zero: every lookup that the
+    reference would make outside the image clamps, on the GPU as in the oracle."""
+    print("case texture limits", flush=True)
+    rng = np.random.default_rng(3)
+    texs = [S.Texture(rng.integers(0, 256, (h, w, 4), dtype=np.uint8)) for w, h in ((1, 1), (32767, 1), (1, 32767))]
+    meshes = []
+    for k, t in enumerate(texs):
+        v, f = S.quad(1.0, 0.0)
+        uv = np.array([[-3.0, -2.5], [4.5, -3.0], [5.0, 6.0], [-2.0, 4.0]], np.float32) * (k + 1)
+        o2w = (S.translate((-2.3 + 2.3 * k, 0.0, 0.0))).astype(np.float32)
+        meshes.append(S.MeshData(v, f, object_to_world=o2w, texture=t, uv=uv))
+    cam = S.Camera(S.look_at((0.0, 4.0, -3.0), (0.0, 0.0, 0.0)), 60.0, 96, 64)
+    sky = (200.0, 210.0, 220.0)
+    ref = O.OracleScene(meshes, sky).render(cam, 2, 3, seed=4)
+    sc = Scene(dev, meshes, sky)
+    assert _same(render(sc, cam, 2, 3, 4), ref)
+    assert _same(render_batch([(sc, cam, 2, 3, 4)])[0], ref)
+    assert len(np.unique(ref.reshape(-1, 3), axis=0)) > 50
+    sc.close()
