@@ -43,9 +43,13 @@
 //     reference for single-mesh scenes and for multi-mesh scenes whenever the
 //     faces tested after the hit belong to the hit mesh.
 //
-// Parity pins: see tests/test_oracle_*.py (reference golden fixtures from the
+// Parity pins (tests/): test_oracle.py (reference golden fixtures from the
 // compiled reference, the reference's own unit-test cases, closed forms,
-// brute force, Philox known-answer vectors).
+// brute force, Philox known-answer vectors), test_glossy.py (the reference's
+// GlossyBRDF::sample on its own recorded draws), test_textures.py (the
+// reference's texture lookups and films, golden tex_probe_crafted.npz),
+// test_pbrt_ingest.py / test_pbrt_reader.py (films and trees from the
+// reference's parser).
 
 #include <algorithm>
 #include <atomic>
