@@ -345,6 +345,16 @@ def main():
     mrays = n_inc / (ms_per_step / 1e3) / 1e6
     achieved_gbs = mean_b * n_inc / (launch_ms / 1e3) / 1e9
 
+    # PCIe-inclusive rate, not the metric: the host-buffer entry point on the
+    # same rays (24 B of ray in, 24 B of hit out, through pageable host memory)
+    host_rays = np.ascontiguousarray(rays[n_coh:])
+    sc.intersect(host_rays)
+    th = time.perf_counter()
+    for _ in range(3):
+        sc.intersect(host_rays)
+    host_mrays = round(3 * n_inc / (time.perf_counter() - th) / 1e6, 2)
+    del host_rays
+
     traffic = None
     tf = ROOT / "profiles" / "closest_hit_traffic.json"
     if tf.exists():
@@ -381,6 +391,7 @@ def main():
                      "bytes_per_ray": round(mean_b, 1), "kernel": "k_closest_hit",
                      "launch_ms": round(launch_ms, 4)},
         "coherent_mrays_per_s": coherent_mrays,
+        "host_buffers_mrays_per_s": host_mrays,
         "tuning": dev.get_tuning(),
         "cpu_baseline": cpu,
     }
