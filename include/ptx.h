@@ -277,7 +277,9 @@ typedef struct ptx_job_status {
 
 int ptx_server_create(ptx_device *dev, const ptx_server_config *cfg, ptx_server **out);
 void ptx_server_destroy(ptx_server *srv);
-/* Queues a job; scene must outlive it.  Rejects with PTX_ERR_BUDGET a job that
+/* Queues a job.  The job holds a reference to the scene until it leaves the
+ * server, so the caller may destroy its scene handle right after submitting.
+ * Rejects with PTX_ERR_BUDGET a job that
  * could never fit the budget. */
 int ptx_server_submit(ptx_server *srv, ptx_scene *scene, const ptx_job_desc *job, uint64_t *job_id);
 int ptx_server_status(ptx_server *srv, uint64_t job_id, ptx_job_status *out);

@@ -324,12 +324,14 @@ class Server:
         self.handle = h
         self.device = device
         self._dims = {}
+        self._scenes = {}       # job id -> Scene, kept until the job is waited for
 
     def submit(self, scene: Scene, cam: Camera, spp: int, max_depth: int, seed: int, nee: bool = False) -> int:
         job = job_desc(cam, spp, max_depth, seed, nee)
         jid = C.c_uint64()
         check(load().ptx_server_submit(self.handle, scene.handle, C.byref(job), C.byref(jid)))
         self._dims[jid.value] = (cam.height, cam.width)
+        self._scenes[jid.value] = scene
         return jid.value
 
     def status(self, job_id: int) -> dict:
@@ -344,6 +346,7 @@ class Server:
         film = np.zeros((h, w, 3), np.float32)
         st = _abi.JobStatus()
         rc = load().ptx_server_wait(self.handle, job_id, film.ctypes.data, C.byref(st))
+        self._scenes.pop(job_id, None)
         if rc != _abi.PTX_OK:
             err = PtxError(rc, load().ptx_last_error().decode(errors="replace"))
             err.status = _status_dict(st)

@@ -60,16 +60,26 @@ int device_usable(const ptx_device *dev) {
    return PTX_OK;
 }
 
-uint32_t lds_depth_of(const ptx_device *dev) { return dev->tuning.lds_stack ? dev->tuning.lds_stack : 12; }
+ptx_tuning tuning_of(const ptx_device *dev) {
+   std::lock_guard<std::mutex> lock(dev->tuning_mu);
+   return dev->tuning;
+}
 
-uint32_t leaf_lanes_of(const ptx_device *dev, bool render) {
-   if (dev->tuning.leaf_lanes) return std::min(64u, dev->tuning.leaf_lanes);
+void set_tuning(ptx_device *dev, const ptx_tuning &t) {
+   std::lock_guard<std::mutex> lock(dev->tuning_mu);
+   dev->tuning = t;
+}
+
+uint32_t lds_depth_of(const ptx_tuning &t) { return t.lds_stack ? t.lds_stack : 12; }
+
+uint32_t leaf_lanes_of(const ptx_tuning &t, bool render) {
+   if (t.leaf_lanes) return std::min(64u, t.leaf_lanes);
    return render ? kDefaultRenderLeafLanes : kDefaultLeafLanes;
 }
 
-uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds, bool extra) {
+uint32_t closest_hit_grid(const ptx_device *dev, const ptx_tuning &t, uint32_t lds, bool extra) {
    int per_cu = closest_hit_blocks_per_cu(lds, extra);
-   if (dev->tuning.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)dev->tuning.blocks_per_cu);
+   if (t.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)t.blocks_per_cu);
    return (uint32_t)(dev->num_cus * per_cu);
 }
 
@@ -78,20 +88,30 @@ size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds
    return 256 + spill_entries * sizeof(uint4);
 }
 
+void scene_retain(ptx_scene *s) { s->refs.fetch_add(1); }
+
+void scene_release(ptx_scene *s) {
+   if (s->refs.fetch_sub(1) == 1) {
+      ptx_device *dev = s->dev;
+      delete s;
+      device_release(dev);
+   }
+}
+
 // Closest-hit launch with stack-spill scratch taken from the stream-ordered allocator.
 int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
-                    ptx_hit_frame *d_frames, hipStream_t stream) {
+                    ptx_hit_frame *d_frames, hipStream_t stream, const ptx_tuning &t) {
    ptx_device *dev = s->dev;
-   const uint32_t lds = lds_depth_of(dev);
-   uint32_t grid = closest_hit_grid(dev, lds, d_counters != nullptr || d_frames != nullptr);
+   const uint32_t lds = lds_depth_of(t);
+   uint32_t grid = closest_hit_grid(dev, t, lds, d_counters != nullptr || d_frames != nullptr);
    const uint32_t needed_blocks = (n + 255) / 256;
    grid = std::max(1u, std::min(grid, needed_blocks));
    void *scratch = nullptr;
    PTX_HIP_TRY(hipMallocAsync(&scratch, closest_hit_scratch_bytes(s, grid, lds), stream));
    uint32_t *work = static_cast<uint32_t *>(scratch);
    uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(scratch) + 256);
-   const uint32_t refill = dev->tuning.refill_lanes ? std::min(64u, dev->tuning.refill_lanes) : 1u;
-   const uint32_t leaf = leaf_lanes_of(dev);
+   const uint32_t refill = t.refill_lanes ? std::min(64u, t.refill_lanes) : 1u;
+   const uint32_t leaf = leaf_lanes_of(t);
    const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill, leaf,
                                      stream);
    const hipError_t fe = hipFreeAsync(scratch, stream);
@@ -103,6 +123,12 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
 }  // namespace ptx
 
 ptx_device::~ptx_device() = default;
+
+ptx_scene::~ptx_scene() {
+   if (!allocations.empty() && dev) (void)hipSetDevice(dev->ordinal);
+   for (void *p : allocations) (void)hipFree(p);
+   allocations.clear();
+}
 
 using namespace ptx;
 
@@ -188,8 +214,21 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
                                                      " indexes past its texture coordinates");
       }
    }
+   {
+      // the texel pool is indexed by u32: each texture that a mesh uses, once
+      std::vector<bool> used(desc->num_textures, false);
+      uint64_t pool = 0;
+      for (uint32_t i = 0; i < desc->num_meshes; i++) {
+         const ptx_mesh_desc &m = desc->meshes[i];
+         if (m.texture == 0 || m.is_light || used[m.texture - 1]) continue;
+         used[m.texture - 1] = true;
+         pool += uint64_t(desc->textures[m.texture - 1].width) * desc->textures[m.texture - 1].height;
+      }
+      if (pool > 0xFFFFFFFFull) return set_error(PTX_ERR_INVALID, "a scene's textures exceed 2^32 texels");
+   }
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    const double t0 = now_s();
+   // a scene that fails part-way frees what it uploaded (~ptx_scene)
    auto s = std::make_unique<ptx_scene>();
    s->dev = dev;
    s->num_meshes = desc->num_meshes;
@@ -279,12 +318,7 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
 
 void ptx_scene_destroy(ptx_scene *scene) {
    if (!scene) return;
-   ptx_device *dev = scene->dev;
-   (void)hipSetDevice(dev->ordinal);
-   for (void *p : scene->allocations) (void)hipFree(p);
-   scene->allocations.clear();
-   delete scene;
-   device_release(dev);
+   scene_release(scene);       // server jobs still holding the scene keep it until they leave
 }
 
 int ptx_scene_get_info(const ptx_scene *scene, ptx_scene_info *out) {
@@ -327,7 +361,8 @@ int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t
    if (!scene || (n && (!d_rays || !d_hits))) return set_error(PTX_ERR_INVALID, "NULL argument");
    if (int rc = device_usable(scene->dev)) return rc;
    PTX_HIP_TRY(hipSetDevice(scene->dev->ordinal));
-   return run_closest_hit(scene, d_rays, n, d_hits, d_counters, d_frames, static_cast<hipStream_t>(stream));
+   return run_closest_hit(scene, d_rays, n, d_hits, d_counters, d_frames, static_cast<hipStream_t>(stream),
+                          tuning_of(scene->dev));
 }
 
 int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_hit *hits,
@@ -355,7 +390,7 @@ int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_h
    ptx_hit *d_hits = static_cast<ptx_hit *>(buf.p[1]);
    ptx_ray_counters *d_cnt = static_cast<ptx_ray_counters *>(buf.p[2]);
    PTX_HIP_TRY(hipMemcpyAsync(d_rays, rays, size_t(n) * 24, hipMemcpyHostToDevice, st));
-   const int rc = run_closest_hit(scene, d_rays, n, d_hits, d_cnt, nullptr, st);
+   const int rc = run_closest_hit(scene, d_rays, n, d_hits, d_cnt, nullptr, st, tuning_of(dev));
    if (rc == PTX_OK) {
       PTX_HIP_TRY(hipMemcpyAsync(hits, d_hits, size_t(n) * sizeof(ptx_hit), hipMemcpyDeviceToHost, st));
       if (counters)
@@ -373,14 +408,14 @@ int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
    if (t->render_mode > PTX_RENDER_WAVEFRONT) return set_error(PTX_ERR_INVALID, "render_mode must be 0, 1 or 2");
    if (t->shade_lanes > 64) return set_error(PTX_ERR_INVALID, "shade_lanes must be 0..64");
    if (t->leaf_lanes > 64) return set_error(PTX_ERR_INVALID, "leaf_lanes must be 0..64");
-   dev->tuning = *t;
+   set_tuning(dev, *t);
    return PTX_OK;
 }
 
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t) {
    if (!dev || !t) return set_error(PTX_ERR_INVALID, "NULL argument");
-   *t = dev->tuning;
-   if (!t->lds_stack) t->lds_stack = lds_depth_of(dev);
+   *t = tuning_of(dev);
+   if (!t->lds_stack) t->lds_stack = lds_depth_of(*t);
    if (!t->block_size) t->block_size = 256;
    if (!t->refill_lanes) t->refill_lanes = 1;
    // leaf_lanes stays 0 when unset: each kernel family then uses its own default

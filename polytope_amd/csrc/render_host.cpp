@@ -87,6 +87,7 @@ void RenderEngine::release() {
    dfree(nee_scratch);
    dfree(d_jobs); dfree(d_scenes); dfree(d_chunks);
    cap = qcap = 0;
+   jobs_cap = scenes_cap = chunks_cap = 0;
    spill_entries = 0;
    graphs_stale = true;
 }
@@ -97,7 +98,7 @@ size_t RenderEngine::bytes_for(uint32_t paths, bool queues) {
 }
 
 bool RenderEngine::regen_round(uint32_t nchunks, uint32_t nscenes) const {
-   return dev->tuning.render_mode != PTX_RENDER_WAVEFRONT && nchunks <= paths_max_chunks() &&
+   return tun.render_mode != PTX_RENDER_WAVEFRONT && nchunks <= paths_max_chunks() &&
           nscenes <= paths_max_scenes();
 }
 
@@ -137,22 +138,29 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       stack_cap = max_stack;
       graphs_stale = true;
    }
+   // a table's capacity is recorded only once its allocation succeeded
    if (njobs > jobs_cap) {
       dfree(d_jobs);
-      jobs_cap = std::max(njobs, 64u);
-      if ((rc = dalloc(&d_jobs, jobs_cap))) return rc;
+      jobs_cap = 0;
+      const uint32_t want = std::max(njobs, 64u);
+      if ((rc = dalloc(&d_jobs, want))) return rc;
+      jobs_cap = want;
       graphs_stale = true;
    }
    if (nscenes > scenes_cap) {
       dfree(d_scenes);
-      scenes_cap = std::max(nscenes, 16u);
-      if ((rc = dalloc(&d_scenes, scenes_cap))) return rc;
+      scenes_cap = 0;
+      const uint32_t want = std::max(nscenes, 16u);
+      if ((rc = dalloc(&d_scenes, want))) return rc;
+      scenes_cap = want;
       graphs_stale = true;
    }
    if (nchunks > chunks_cap) {
       dfree(d_chunks);
-      chunks_cap = std::max(nchunks, 1024u);
-      if ((rc = dalloc(&d_chunks, chunks_cap))) return rc;
+      chunks_cap = 0;
+      const uint32_t want = std::max(nchunks, 1024u);
+      if ((rc = dalloc(&d_chunks, want))) return rc;
+      chunks_cap = want;
    }
    return PTX_OK;
 }
@@ -171,8 +179,8 @@ int RenderEngine::ensure_queues(uint32_t need) {
       qcap = need;
       graphs_stale = true;
    }
-   if (graph_leaf != leaf_lanes_of(dev, true)) {
-      graph_leaf = leaf_lanes_of(dev, true);
+   if (graph_leaf != leaf_lanes_of(tun, true)) {
+      graph_leaf = leaf_lanes_of(tun, true);
       graphs_stale = true;
    }
    if (graphs_stale || !g_ab) {
@@ -188,7 +196,7 @@ int RenderEngine::ensure_queues(uint32_t need) {
          float *qin = dir == 0 ? qa : qb, *qout = dir == 0 ? qb : qa;
          uint32_t *cin = words + (dir == 0 ? kCountA : kCountB), *cout = words + (dir == 0 ? kCountB : kCountA);
          launch_bounce(qin, cin, qout, cout, hits, d_jobs, d_scenes, res, words + kWork, spill, stack_cap,
-                       leaf_lanes_of(dev, true), words + kBounce, nee ? shadow_args() : ShadowArgs{}, ext_grid,
+                       leaf_lanes_of(tun, true), words + kBounce, nee ? shadow_args() : ShadowArgs{}, ext_grid,
                        shade_grid, stream);
          PTX_HIP_TRY(hipStreamEndCapture(stream, &g));
          hipGraphExec_t ex = nullptr;
@@ -238,13 +246,19 @@ std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jo
 }
 
 int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
-   int rc;
-   if ((rc = dalloc(&J.film, (size_t)npix * 3)) || (rc = dalloc(&J.primary, (size_t)npix * 16))) {
-      dfree(J.film);
-      return rc;
+   // stream-ordered: a job's buffers come and go between the rounds that use
+   // them without waiting for the device
+   J.film = nullptr;
+   J.primary = nullptr;
+   hipError_t e = hipMallocAsync((void **)&J.film, (size_t)npix * 3 * sizeof(float), stream);
+   if (e == hipSuccess) e = hipMallocAsync((void **)&J.primary, (size_t)npix * 16 * sizeof(uint32_t), stream);
+   if (e != hipSuccess) {
+      (void)hipGetLastError();
+      free_job(J);
+      return set_error(PTX_ERR_OOM, std::string("hipMallocAsync (job film): ") + hipGetErrorString(e));
    }
    launch_zero_floats(J.film, (size_t)npix * 3, stream);
-   launch_primary(S, J, words + kWork, spill, leaf_lanes_of(dev, true), std::min<uint32_t>(ext_grid, (npix + 255) / 256),
+   launch_primary(S, J, words + kWork, spill, leaf_lanes_of(tun, true), std::min<uint32_t>(ext_grid, (npix + 255) / 256),
                   stream);
    return PTX_OK;
 }
@@ -258,9 +272,9 @@ int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint3
       if (rc) return rc;
    }
    if (regen_round((uint32_t)r.size(), nscenes)) {
-      const uint32_t shade_min = dev->tuning.shade_lanes ? dev->tuning.shade_lanes : kDefaultShadeLanes;
+      const uint32_t shade_min = tun.shade_lanes ? tun.shade_lanes : kDefaultShadeLanes;
       launch_paths(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, nscenes, res, words + kWork, spill, stack_cap,
-                   shade_min, leaf_lanes_of(dev, true), nee ? nee_scratch : nullptr, glossy, textured,
+                   shade_min, leaf_lanes_of(tun, true), nee ? nee_scratch : nullptr, glossy, textured,
                    std::min<uint32_t>(paths_grid, (nslots + 255) / 256), stream);
       launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, nullptr, shade_grid, stream);
       return PTX_OK;
@@ -271,8 +285,14 @@ int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint3
    launch_round_start(d_round, (uint32_t)r.size(), nslots, d_jobs, d_scenes, res, qa, words + kCountA,
                       words + kBounce, sh, spill, stack_cap, std::min<uint32_t>(shade_grid, (nslots + 255) / 256),
                       stream);
-   for (uint32_t b = 1; b <= depth; b++)
-      (void)hipGraphLaunch(b & 1 ? (nee ? g_ab_nee : g_ab) : (nee ? g_ba_nee : g_ba), stream);
+   for (uint32_t b = 1; b <= depth; b++) {
+      const hipError_t e = hipGraphLaunch(b & 1 ? (nee ? g_ab_nee : g_ab) : (nee ? g_ba_nee : g_ba), stream);
+      if (e != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(PTX_ERR_HIP, std::string("hipGraphLaunch (bounce ") + std::to_string(b) +
+                                             "): " + hipGetErrorString(e));
+      }
+   }
    launch_accumulate(d_round, (uint32_t)r.size(), d_jobs, res, nee ? direct : nullptr, shade_grid, stream);
    return PTX_OK;
 }
@@ -285,12 +305,14 @@ int RenderEngine::finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film
 }
 
 void RenderEngine::free_job(JobDev &J) {
-   dfree(J.film);
-   dfree(J.primary);
+   if (J.film) (void)hipFreeAsync(J.film, stream);
+   if (J.primary) (void)hipFreeAsync(J.primary, stream);
+   J.film = nullptr;
+   J.primary = nullptr;
 }
 
 uint32_t RenderEngine::capacity_for(uint64_t total_paths, uint32_t max_npix, bool regen) const {
-   uint32_t want = dev->tuning.paths_per_wave ? dev->tuning.paths_per_wave : regen ? kDefaultRegenPaths : kDefaultPaths;
+   uint32_t want = tun.paths_per_wave ? tun.paths_per_wave : regen ? kDefaultRegenPaths : kDefaultPaths;
    want = (uint32_t)std::min<uint64_t>(want, std::max<uint64_t>(total_paths, max_npix));
    // regeneration keeps using what is allocated; wavefront queues are sized per call
    return std::max(std::max(want, max_npix), regen ? cap : 0u);
@@ -331,10 +353,20 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
    std::vector<Chunk> all;
    all.reserve(nchunks);
    for (const auto &r : rounds) all.insert(all.end(), r.begin(), r.end());
-   PTX_HIP_TRY(hipMemcpyAsync(d_jobs, jd.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice, st));
-   PTX_HIP_TRY(hipMemcpyAsync(d_scenes, sd.data(), sd.size() * sizeof(DevScene), hipMemcpyHostToDevice, st));
-   if (!all.empty())
-      PTX_HIP_TRY(hipMemcpyAsync(d_chunks, all.data(), all.size() * sizeof(Chunk), hipMemcpyHostToDevice, st));
+   {
+      // the tables go up before any round; a failed copy still frees every job's buffers
+      hipError_t e = hipMemcpyAsync(d_jobs, jd.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess)
+         e = hipMemcpyAsync(d_scenes, sd.data(), sd.size() * sizeof(DevScene), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess && !all.empty())
+         e = hipMemcpyAsync(d_chunks, all.data(), all.size() * sizeof(Chunk), hipMemcpyHostToDevice, st);
+      if (e != hipSuccess) {
+         (void)hipGetLastError();
+         (void)hipStreamSynchronize(st);
+         cleanup();
+         return set_error(PTX_ERR_HIP, std::string("render batch tables: ") + hipGetErrorString(e));
+      }
+   }
    size_t off = 0;
    for (const auto &r : rounds) {
       uint32_t depth = 0;
@@ -409,7 +441,9 @@ int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb) {
    if ((rc = device_usable(dev))) return rc;
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    std::lock_guard<std::mutex> lock(dev->mu);
-   return engine_of(dev)->run_plain(scene, *job, film_rgb);
+   RenderEngine *eng = engine_of(dev);
+   eng->tun = tuning_of(dev);
+   return eng->run_plain(scene, *job, film_rgb);
 }
 
 int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_t n_jobs, float *const *films) {
@@ -437,7 +471,9 @@ int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_
    }
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    std::lock_guard<std::mutex> lock(dev->mu);
-   return engine_of(dev)->run_batch(rj, uniq, out);
+   RenderEngine *eng = engine_of(dev);
+   eng->tun = tuning_of(dev);
+   return eng->run_batch(rj, uniq, out);
 }
 
 }  // extern "C"
@@ -468,8 +504,12 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
    if (!dev || !scene) return set_error(PTX_ERR_INVALID, "NULL argument");
    if (int rc = device_usable(dev)) return rc;
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
-   const ptx_tuning saved = dev->tuning;
+   // Trials run on local tunings (the ray queries take theirs as an argument,
+   // the render trials set the engine's); the device's tuning is written once,
+   // at the end, so calls on other threads never see a candidate.
+   const ptx_tuning saved = tuning_of(dev);
    ptx_tuning best = saved;
+   ptx_tuning trial = saved;
    // 1. traversal: LDS stack depth x refill threshold on 2^20 incoherent rays in the root box
    const uint32_t n = 1u << 20;
    std::vector<float> rays((size_t)n * 6);
@@ -502,12 +542,12 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
       std::lock_guard<std::mutex> lock(dev->mu);
       for (uint32_t lds : {8u, 12u, 16u})
          for (uint32_t refill : {1u, 8u, 32u}) {
-            dev->tuning.lds_stack = lds;
-            dev->tuning.refill_lanes = refill;
+            trial.lds_stack = lds;
+            trial.refill_lanes = refill;
             float ms_min = 1e30f;
             for (int rep = 0; rep < 3 && rc == PTX_OK; rep++) {
                (void)hipEventRecord(e0, dev->stream);
-               rc = run_closest_hit(scene, d_rays, n, d_hits, nullptr, nullptr, dev->stream);
+               rc = run_closest_hit(scene, d_rays, n, d_hits, nullptr, nullptr, dev->stream, trial);
                (void)hipEventRecord(e1, dev->stream);
                (void)hipEventSynchronize(e1);
                float ms = 0;
@@ -523,7 +563,7 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
    }
    (void)hipFree(d_rays);
    (void)hipFree(d_hits);
-   dev->tuning = best;
+   trial = best;
    // 2. round form and path slots per round, on a probe render of the scene
    if (rc == PTX_OK) {
       ptx_job_desc job{};
@@ -548,14 +588,16 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
       double best_s = 1e30;
       uint32_t best_paths = best.paths_per_wave, best_mode = best.render_mode;
       std::lock_guard<std::mutex> lock(dev->mu);
+      RenderEngine *eng = engine_of(dev);
       for (uint32_t mode : {(uint32_t)PTX_RENDER_REGEN, (uint32_t)PTX_RENDER_WAVEFRONT})
          for (uint32_t paths : {1u << 20, 1u << 21, 1u << 22, 1u << 23}) {
-            dev->tuning.paths_per_wave = paths;
-            dev->tuning.render_mode = mode;
+            trial.paths_per_wave = paths;
+            trial.render_mode = mode;
+            eng->tun = trial;
             double t_min = 1e30;
             for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
                const double t0 = now_s();
-               rc = engine_of(dev)->run_batch(rj, {scene}, films);
+               rc = eng->run_batch(rj, {scene}, films);
                t_min = std::min(t_min, now_s() - t0);
             }
             if (rc == PTX_OK && t_min < best_s) {
@@ -568,15 +610,16 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
       best.render_mode = best_mode;
       // 3. regeneration: lanes a wave gathers before it shades
       if (best_mode == PTX_RENDER_REGEN) {
-         dev->tuning.paths_per_wave = best_paths;
-         dev->tuning.render_mode = best_mode;
+         trial.paths_per_wave = best_paths;
+         trial.render_mode = best_mode;
          uint32_t best_shade = 0;
          for (uint32_t shade : {1u, 8u, 16u, 32u, 48u}) {
-            dev->tuning.shade_lanes = shade;
+            trial.shade_lanes = shade;
+            eng->tun = trial;
             double t_min = 1e30;
             for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
                const double t0 = now_s();
-               rc = engine_of(dev)->run_batch(rj, {scene}, films);
+               rc = eng->run_batch(rj, {scene}, films);
                t_min = std::min(t_min, now_s() - t0);
             }
             if (rc == PTX_OK && t_min < best_s) {
@@ -586,14 +629,15 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
          }
          best.shade_lanes = best_shade;
          // 4. lanes a traversal step gathers at leaves (render kernels)
-         dev->tuning.shade_lanes = best_shade;
+         trial.shade_lanes = best_shade;
          uint32_t best_leaf = 0;
          for (uint32_t leaf : {1u, 8u, 16u, 32u}) {
-            dev->tuning.leaf_lanes = leaf;
+            trial.leaf_lanes = leaf;
+            eng->tun = trial;
             double t_min = 1e30;
             for (int rep = 0; rep < 2 && rc == PTX_OK; rep++) {
                const double t0 = now_s();
-               rc = engine_of(dev)->run_batch(rj, {scene}, films);
+               rc = eng->run_batch(rj, {scene}, films);
                t_min = std::min(t_min, now_s() - t0);
             }
             if (rc == PTX_OK && t_min < best_s) {
@@ -606,7 +650,8 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
    }
    (void)hipEventDestroy(e0);
    (void)hipEventDestroy(e1);
-   dev->tuning = rc == PTX_OK ? best : saved;
-   if (chosen) *chosen = dev->tuning;
+   const ptx_tuning result = rc == PTX_OK ? best : saved;
+   set_tuning(dev, result);
+   if (chosen) *chosen = result;
    return rc;
 }

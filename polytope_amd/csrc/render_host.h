@@ -32,6 +32,9 @@ struct RenderEngine {
 
    ptx_device *dev = nullptr;
    hipStream_t stream = nullptr;
+   // The tuning this engine runs with: the device's, copied at the start of a
+   // ptx_render / ptx_render_batch call, or once when a server is created.
+   ptx_tuning tun{};
    uint32_t cap = 0;                        // path slots (res)
    uint32_t qcap = 0;                       // path slots of the wavefront queues (0 = none)
    float *qa = nullptr, *qb = nullptr;      // queues, 16 words per entry
@@ -62,7 +65,7 @@ struct RenderEngine {
    // the wavefront form may run.
    static size_t bytes_for(uint32_t paths, bool queues);
    // Whether a round of nchunks chunks over nscenes scenes runs as one
-   // path-regeneration launch under the device's tuning.
+   // path-regeneration launch under the engine's tuning.
    bool regen_round(uint32_t nchunks, uint32_t nscenes) const;
    int ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uint32_t nscenes, uint32_t nchunks);
    // Wavefront queues and hit records for at least `need` slots, and the

@@ -44,19 +44,28 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
 
 double now_s();
 const char *last_error_cstr();
-uint32_t lds_depth_of(const ptx_device *dev);
+// The device's tuning, copied under its lock: every launch works from such a
+// snapshot, so ptx_set_tuning on another thread never changes a call in flight.
+ptx_tuning tuning_of(const ptx_device *dev);
+void set_tuning(ptx_device *dev, const ptx_tuning &t);
+uint32_t lds_depth_of(const ptx_tuning &t);
 // Lanes of a wave that gather at leaves before the wave runs a leaf step:
 // measured best 1 for ray queries (leaf and interior steps are balanced) and
 // 16 for the render kernels (whose leaf and shading steps share the wave).
 constexpr uint32_t kDefaultLeafLanes = 1;
 constexpr uint32_t kDefaultRenderLeafLanes = 16;
-uint32_t leaf_lanes_of(const ptx_device *dev, bool render = false);
-uint32_t closest_hit_grid(const ptx_device *dev, uint32_t lds, bool extra);
+uint32_t leaf_lanes_of(const ptx_tuning &t, bool render = false);
+uint32_t closest_hit_grid(const ptx_device *dev, const ptx_tuning &t, uint32_t lds, bool extra);
 size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds);
 int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
-                    ptx_hit_frame *d_frames, hipStream_t stream);
+                    ptx_hit_frame *d_frames, hipStream_t stream, const ptx_tuning &t);
 void device_retain(ptx_device *dev);
 void device_release(ptx_device *dev);
+// Scenes are reference-counted: ptx_scene_create hands out one reference,
+// every server job holds one until it leaves the server, and the device
+// buffers are freed when the last goes.
+void scene_retain(ptx_scene *s);
+void scene_release(ptx_scene *s);
 // PTX_ERR_INVALID with a message when dev is NULL or closed, else PTX_OK.
 int device_usable(const ptx_device *dev);
 
@@ -71,6 +80,7 @@ struct ptx_device {
    int num_cus = 0;
    std::string arch;
    hipStream_t stream = nullptr;     // library stream (default priority)
+   mutable std::mutex tuning_mu;     // guards tuning (see tuning_of)
    ptx_tuning tuning{};
    std::mutex mu;                            // serialises use of the library stream
    std::unique_ptr<ptx::RenderEngine> engine;  // path queues and graphs, created on first render
@@ -78,6 +88,7 @@ struct ptx_device {
 };
 
 struct ptx_scene {
+   std::atomic<int> refs{1};
    ptx_device *dev = nullptr;
    ptx::HostBvh bvh;                 // reference-layout BVH (kept for ptx_scene_copy_bvh)
    ptx::DevScene d{};                // device pointers + root
@@ -86,4 +97,5 @@ struct ptx_scene {
    uint32_t num_meshes = 0;
    bool has_glossy = false;          // a mesh that is not a light has PTX_BRDF_GLOSSY
    bool has_texture = false;         // a Lambert mesh that is not a light has a texture
+   ~ptx_scene();                     // frees every device buffer in `allocations`
 };

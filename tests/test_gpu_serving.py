@@ -139,3 +139,79 @@ def test_autotune_keeps_results(dev):
     after = render_batch([(sb, S.Camera(S.look_at((0.3, 0.5, -3.0), (0, 0, 0)), 50.0, 40, 30), 4, 4, 3)])[0]
     assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
     dev.set_tuning()
+
+
+def test_server_job_keeps_its_scene(dev):
+    # the caller closes its scene handle right after submitting: the jobs hold
+    # their own reference, and their films equal the same jobs rendered alone
+    _, cam_a, _, meshes_b = _scenes(dev)
+    cam_b = S.Camera(S.look_at((0.3, 0.5, -3.0), (0.0, 0.0, 0.0)), 50.0, 48, 36)
+    keep = Scene(dev, meshes_b, (100.0, 120.0, 150.0))
+    alone = [render_batch([(keep, cam_b, 5 + i, 4, 70 + i)])[0] for i in range(3)]
+    keep.close()
+    srv = Server(dev, hbm_budget_bytes=4 << 30, paths_per_wave=20000)
+    try:
+        tmp = Scene(dev, meshes_b, (100.0, 120.0, 150.0))
+        ids = [srv.submit(tmp, cam_b, 5 + i, 4, 70 + i) for i in range(3)]
+        tmp.close()
+        del tmp
+        for jid, a in zip(ids, alone):
+            film, st = srv.wait(jid)
+            assert st["state"] == _abi.JOB_DONE
+            assert np.array_equal(film.view(np.uint32), a.view(np.uint32))
+    finally:
+        srv.close()
+
+
+def test_server_serves_every_job_when_the_pool_is_full(dev):
+    # ten 64x48 jobs on a 20,000-slot pool: a round holds six of them, so the
+    # rest must be first in line for the next round; by the time the first job
+    # has its eight samples every job has been served
+    sa, cam_a, _, _ = _scenes(dev)
+    srv = Server(dev, hbm_budget_bytes=4 << 30, paths_per_wave=20000)
+    try:
+        ids = [srv.submit(sa, cam_a, 8, 3, 90 + i) for i in range(10)]
+        first, _ = srv.wait(ids[0])
+        for jid in ids[1:]:
+            st = srv.status(jid)
+            assert st["samples_done"] > 0, st
+        films = [first] + [srv.wait(jid)[0] for jid in ids[1:]]
+    finally:
+        srv.close()
+    for i in (0, 6, 9):
+        (alone,) = render_batch([(sa, cam_a, 8, 3, 90 + i)])
+        assert np.array_equal(alone.view(np.uint32), films[i].view(np.uint32))
+
+
+def test_batch_recovers_after_an_allocation_failure(dev):
+    # a batch whose path pool cannot be allocated fails with PTX_ERR_OOM; once
+    # memory is back, the engine's tables are allocated again and the next batch
+    # renders bit-identically
+    sa, cam_a, _, _ = _scenes(dev)
+    job = (sa, cam_a, 3, 3, 61)
+    dev.set_tuning(paths_per_wave=1 << 16)
+    (before,) = render_batch([job])
+    free, _ = torch.cuda.mem_get_info()
+    print(f"case allocation failure: filling {free / 2**30:.1f} GiB but about 1 GiB", flush=True)
+    hog = None
+    for spare in (1 << 30, 3 << 29, 1 << 31, 5 << 29):       # leave less than the 3.2 GB pool
+        try:
+            hog = torch.empty(max(0, free - spare), dtype=torch.uint8, device="cuda")
+            break
+        except RuntimeError:
+            continue
+    if hog is None:
+        pytest.skip("could not fill device memory")
+    try:
+        dev.set_tuning(paths_per_wave=1 << 28)          # 3.2 GB of path results
+        big = S.Camera(cam_a.camera_to_world, 60.0, 1024, 1024)
+        with pytest.raises(PtxError) as e:
+            render_batch([(sa, big, 256, 2, 1)])
+        assert e.value.code == _abi.PTX_ERR_OOM
+    finally:
+        del hog
+        torch.cuda.empty_cache()
+        dev.set_tuning(paths_per_wave=1 << 16)
+    (after,) = render_batch([job])
+    dev.set_tuning()
+    assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
