@@ -157,6 +157,7 @@ int ptx_device_open(int ordinal, ptx_device **out) {
    d->num_cus = prop.multiProcessorCount;
    d->arch = arch;
    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
       delete d;
       return set_error(PTX_ERR_HIP, "hipStreamCreate failed");
    }

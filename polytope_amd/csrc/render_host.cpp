@@ -31,7 +31,11 @@ int dalloc(T **p, size_t n) {
    *p = nullptr;
    if (n == 0) return PTX_OK;
    const hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-   if (e != hipSuccess) return set_error(PTX_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+   if (e != hipSuccess) {
+      (void)hipGetLastError();      // the caller's own later HIP calls must not see this failure
+      *p = nullptr;
+      return set_error(PTX_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+   }
    return PTX_OK;
 }
 
@@ -202,7 +206,10 @@ int RenderEngine::ensure_queues(uint32_t need) {
          hipGraphExec_t ex = nullptr;
          const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
          (void)hipGraphDestroy(g);
-         if (ie != hipSuccess) return set_error(PTX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+         if (ie != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(PTX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+         }
          (nee ? (dir == 0 ? g_ab_nee : g_ba_nee) : (dir == 0 ? g_ab : g_ba)) = ex;
       }
       graphs_stale = false;
@@ -529,6 +536,7 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
    ptx_hit *d_hits = nullptr;
    PTX_HIP_TRY(hipMalloc((void **)&d_rays, rays.size() * sizeof(float)));
    if (hipMalloc((void **)&d_hits, (size_t)n * sizeof(ptx_hit)) != hipSuccess) {
+      (void)hipGetLastError();
       (void)hipFree(d_rays);
       return set_error(PTX_ERR_OOM, "autotune buffers");
    }
