@@ -215,6 +215,22 @@ int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t
                                  ptx_hit *d_hits, ptx_ray_counters *d_counters,
                                  ptx_hit_frame *d_frames, void *stream);
 
+/* Same as ptx_intersect_closest_device, but allocates nothing: no
+ * hipMalloc, hipFree or allocating library call.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * traversal's work counters and stack spill live in a static device array of
+ * 64 MiB per GPU, held from ptx_device_open until the process ends; the
+ * launch grid shrinks to what it holds, and a scene whose BVH needs more than
+ * one block per CU can get (deferred stacks deeper than the LDS stack by
+ * about 64 levels) is refused with PTX_ERR_INVALID.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (on different
+ * streams, or from different threads) race; serialise them on one stream.
+ * Asynchronous; results equal ptx_intersect_closest_device's bit for bit. */
+int ptx_intersect_closest_device_noalloc(ptx_scene *scene, const float *d_rays, uint32_t n,
+                                         ptx_hit *d_hits, ptx_ray_counters *d_counters,
+                                         ptx_hit_frame *d_frames, void *stream);
+
 /* Plain GPU render of one job (one launch series per bounce, as the reference
  * CUDA path), film_rgb = W*H*3 floats on the host (BoxFilter::Output values). */
 int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb);

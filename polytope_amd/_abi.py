@@ -123,6 +123,8 @@ SIGNATURES = {
     "ptx_intersect_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "ptx_intersect_closest_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
+    "ptx_intersect_closest_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                                       C.c_void_p, C.c_void_p]),
     "ptx_render": (C.c_int, [C.c_void_p, C.POINTER(JobDesc), C.c_void_p]),
     "ptx_render_batch": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(JobDesc), C.c_uint32,
                                    C.POINTER(C.c_void_p)]),

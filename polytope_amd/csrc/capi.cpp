@@ -161,6 +161,9 @@ int ptx_device_open(int ordinal, ptx_device **out) {
       delete d;
       return set_error(PTX_ERR_HIP, "hipStreamCreate failed");
    }
+   // set-up of the no-allocation entry point: this device's copy of its static
+   // scratch (a failure here only disables that entry point)
+   if (noalloc_scratch_address(ordinal, &d->noalloc_scratch) != 0) d->noalloc_scratch = nullptr;
    *out = d;
    return PTX_OK;
 }
@@ -364,6 +367,43 @@ int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t
    PTX_HIP_TRY(hipSetDevice(scene->dev->ordinal));
    return run_closest_hit(scene, d_rays, n, d_hits, d_counters, d_frames, static_cast<hipStream_t>(stream),
                           tuning_of(scene->dev));
+}
+
+int ptx_intersect_closest_device_noalloc(ptx_scene *scene, const float *d_rays, uint32_t n, ptx_hit *d_hits,
+                                         ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, void *stream) {
+   if (!scene || (n && (!d_rays || !d_hits))) return set_error(PTX_ERR_INVALID, "NULL argument");
+   ptx_device *dev = scene->dev;
+   if (int rc = device_usable(dev)) return rc;
+   if (!dev->noalloc_scratch)
+      return set_error(PTX_ERR_HIP, "the no-allocation scratch of this device could not be located");
+   const ptx_tuning t = tuning_of(dev);
+   const uint32_t lds = lds_depth_of(t);
+   // the spill takes (max_stack - lds) entries of 16 bytes per resident lane;
+   // the grid shrinks to what the scratch holds, down to one block per CU
+   const size_t per_block = scene->d.max_stack > lds ? size_t(scene->d.max_stack - lds) * 256 * sizeof(uint4) : 0;
+   uint32_t grid = closest_hit_grid(dev, t, lds, d_counters != nullptr || d_frames != nullptr);
+   if (per_block) {
+      const size_t fit = (kNoAllocScratchBytes - 256) / per_block;
+      if (fit < (size_t)dev->num_cus)
+         return set_error(PTX_ERR_INVALID,
+                          "scene BVH too deep for the no-allocation scratch: " + std::to_string(scene->d.max_stack) +
+                                " stack levels need " + std::to_string(per_block * dev->num_cus + 256) +
+                                " bytes for one block per CU, the scratch holds " +
+                                std::to_string(kNoAllocScratchBytes) + "; use ptx_intersect_closest_device");
+      grid = (uint32_t)std::min<size_t>(grid, fit);
+   }
+   grid = std::max(1u, std::min(grid, (n + 255) / 256));
+   PTX_HIP_TRY(hipSetDevice(dev->ordinal));
+   uint32_t *work = static_cast<uint32_t *>(dev->noalloc_scratch);
+   uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(dev->noalloc_scratch) + 256);
+   const uint32_t refill = t.refill_lanes ? std::min(64u, t.refill_lanes) : 1u;
+   // the work counters are cleared by a kernel at the head of the launch (no memset)
+   if (launch_closest_hit(scene->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill,
+                          leaf_lanes_of(t), static_cast<hipStream_t>(stream)) != 0) {
+      (void)hipGetLastError();
+      return set_error(PTX_ERR_HIP, "closest-hit launch failed");
+   }
+   return PTX_OK;
 }
 
 int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_hit *hits,

@@ -15,6 +15,13 @@ int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_
 // Resident blocks per CU of the closest-hit kernel; extra = counters or frames requested.
 int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra);
 
+// Size of the per-device static scratch of the no-allocation closest-hit entry
+// point: 256 bytes of work counters, then the deferred-stack spill.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+// Device address of that scratch on device `ordinal`, which must be current.
+// Looked up once per device (hipGetSymbolAddress); 0 on success.
+int noalloc_scratch_address(int ordinal, void **ptr);
+
 }  // namespace ptx
 
 namespace ptx {

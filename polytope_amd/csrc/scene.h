@@ -80,6 +80,7 @@ struct ptx_device {
    int num_cus = 0;
    std::string arch;
    hipStream_t stream = nullptr;     // library stream (default priority)
+   void *noalloc_scratch = nullptr;  // this device's static scratch (ptx_intersect_closest_device_noalloc)
    mutable std::mutex tuning_mu;     // guards tuning (see tuning_of)
    ptx_tuning tuning{};
    std::mutex mu;                            // serialises use of the library stream

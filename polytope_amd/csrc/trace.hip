@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
+#include "launch.h"
 #include "traverse.h"
 
 #pragma clang fp contract(off)
@@ -92,6 +94,36 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 }
 
 }  // namespace dev
+
+// Scratch of ptx_intersect_closest_device_noalloc: the traversal's work
+// counters (first 256 bytes) and its stack spill, one copy per device, held
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel (on the host it is only a stand-in for the symbol): it uses the
+// address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) uint4 scratch[kNoAllocScratchBytes / sizeof(uint4)];
+
+namespace {
+constexpr int kMaxOrdinals = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxOrdinals] = {};
+}  // namespace
+
+int noalloc_scratch_address(int ordinal, void **ptr) {
+   *ptr = nullptr;
+   if (ordinal < 0 || ordinal >= kMaxOrdinals) return -1;
+   std::lock_guard<std::mutex> lock(g_scratch_mu);
+   if (!g_scratch_ptr[ordinal]) {
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+         (void)hipGetLastError();
+         return -1;
+      }
+      g_scratch_ptr[ordinal] = p;
+   }
+   *ptr = g_scratch_ptr[ordinal];
+   return 0;
+}
 
 // Host launcher.  work: >= 8 u32 of device memory; spill sized by the caller
 // for (max_stack - lds) * grid * 256 entries.
