@@ -78,6 +78,8 @@ def lib() -> C.CDLL:
         L.orc_intersect_ids.restype = C.c_uint64
         L.orc_intersect_ids.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
         L.orc_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32]
+        L.orc_render_counted.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32,
+                                         C.c_void_p]
         L.orc_philox4x32_10.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_acos.restype = C.c_float
@@ -144,12 +146,19 @@ class OracleScene:
         return ids, int(s)
 
     def render(self, cam, spp: int, max_depth: int, seed: int, threads: int = 0, row_step: int = 1,
-               nee: bool = False) -> np.ndarray:
+               nee: bool = False, stats: bool = False):
+        """The film (rows y % row_step == 0); with stats, also {rays, boxes,
+        tris, samples}: rays traced and the reference's per-ray counters summed
+        over them."""
         from polytope_amd import camera_desc
         cd = camera_desc(cam)
         job = OrcJob(cam.width, cam.height, spp, max_depth, seed, 1 if nee else 0)
         film = np.zeros((cam.height, cam.width, 3), np.float32)
-        lib().orc_render(self.h, C.byref(cd), C.byref(job), film.ctypes.data, threads or os.cpu_count(), row_step)
+        counts = np.zeros(4, np.uint64)
+        lib().orc_render_counted(self.h, C.byref(cd), C.byref(job), film.ctypes.data, threads or os.cpu_count(),
+                                 row_step, counts.ctypes.data if stats else None)
+        if stats:
+            return film, dict(zip(("rays", "boxes", "tris", "samples"), (int(x) for x in counts)))
         return film
 
     def __del__(self):

@@ -904,8 +904,15 @@ static void add_direct(const Scene& s, const Isect& is, const float refl[3], flo
    }
 }
 
+// Work of a render, for the algorithmic-byte accounting: rays traced (every
+// bounce's closest-hit query) and, summed over them, the reference's per-ray
+// counters (num_bb_hits + num_bb_misses, num_triangle_isects).
+struct RayStats {
+   uint64_t rays = 0, boxes = 0, tris = 0;
+};
+
 static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t pixel_index,
-                      uint32_t sample_index, uint32_t max_depth, uint32_t seed, bool nee) {
+                      uint32_t sample_index, uint32_t max_depth, uint32_t seed, bool nee, RayStats* stats = nullptr) {
    Ray cur = primary;
    uint32_t bounces = 0;
    float sr = 1.f, sg = 1.f, sb = 1.f;
@@ -921,6 +928,11 @@ static Spd get_sample(const Scene& s, const Ray& primary, int x, int y, uint32_t
       is.px = x; is.py = y;
       cur.min_t = kInf;
       intersect(s, cur, is);
+      if (stats) {
+         stats->rays++;
+         stats->boxes += is.bb_hits + is.bb_misses;
+         stats->tris += is.tri_isects;
+      }
       if (!is.hits) {
          Spd spd = {0, 0, 0};
          if (s.has_sky) spd = {s.sky[0] * sr, s.sky[1] * sg, s.sky[2] * sb};
@@ -1151,15 +1163,29 @@ uint64_t orc_intersect_ids(const orc::Scene* s, const float* rays, uint32_t n, u
 // over `threads` workers, BoxFilter accumulation in sample order, then x 1/spp.
 // If `row_step` > 1 only rows y % row_step == 0 are rendered (a deterministic
 // subset for timing); other rows are left as zeros.
+// stats (optional): {rays, boxes, face tests, samples} of the rendered rows.
+void orc_render_counted(const orc::Scene* s, const orc::CameraIn* cam, const orc::JobIn* job, float* film,
+                        int threads, uint32_t row_step, uint64_t* stats);
+
 void orc_render(const orc::Scene* s, const orc::CameraIn* cam, const orc::JobIn* job, float* film,
                 int threads, uint32_t row_step) {
+   orc_render_counted(s, cam, job, film, threads, row_step, nullptr);
+}
+
+void orc_render_counted(const orc::Scene* s, const orc::CameraIn* cam, const orc::JobIn* job, float* film,
+                        int threads, uint32_t row_step, uint64_t* stats) {
    const uint32_t W = job->width, H = job->height;
    orc::Camera c{cam->camera_to_world, cam->one_over_width, cam->one_over_height, cam->aspect_ratio,
                  cam->tan_fov_over_2};
    const uint32_t tiles_x = (W + 1) / 2, tiles_y = (H + 1) / 2;
    std::atomic<uint32_t> next{0};
+   std::mutex stats_mu;
    if (row_step == 0) row_step = 1;
+   if (stats) stats[0] = stats[1] = stats[2] = stats[3] = 0;
    auto worker = [&]() {
+      orc::RayStats mine;
+      uint64_t samples = 0;
+      orc::RayStats* st = stats ? &mine : nullptr;
       while (true) {
          const uint32_t t = next.fetch_add(1);
          if (t >= tiles_x * tiles_y) break;
@@ -1172,7 +1198,8 @@ void orc_render(const orc::Scene* s, const orc::CameraIn* cam, const orc::JobIn*
                const uint32_t pix = y * W + x;
                for (uint32_t sidx = 0; sidx < job->spp; sidx++) {
                   orc::Spd smp = orc::get_sample(*s, primary, (int)x, (int)y, pix, sidx, job->max_depth, job->seed,
-                                                 (job->flags & 1u) != 0);
+                                                 (job->flags & 1u) != 0, st);
+                  samples++;
                   if (smp.r > 255) smp.r = 255;
                   if (smp.g > 255) smp.g = 255;
                   if (smp.b > 255) smp.b = 255;
@@ -1184,6 +1211,13 @@ void orc_render(const orc::Scene* s, const orc::CameraIn* cam, const orc::JobIn*
                film[3 * size_t(pix) + 2] = ab * div;
             }
          }
+      }
+      if (stats) {
+         std::lock_guard<std::mutex> lock(stats_mu);
+         stats[0] += mine.rays;
+         stats[1] += mine.boxes;
+         stats[2] += mine.tris;
+         stats[3] += samples;
       }
    };
    if (threads <= 1) worker();

@@ -369,3 +369,27 @@ def test_next_event_estimation_closed_form():
     assert np.array_equal(film2[lit], film.reshape(-1, 3)[lit])
     # off by default: the same scene without the flag renders black
     assert not O.OracleScene([plane], None, lights=lights).render(cam, 1, 1, 3).any()
+
+
+def test_render_ray_accounting():
+    # the counted render traces one ray per sample per bounce and sums the
+    # per-ray counters of the reference's intersect (checked against the
+    # counters orc_intersect returns for the same primary rays at depth 0)
+    import ctypes as C
+    from polytope_amd import camera_desc
+    meshes, cam, sky = S.plain_render_scene(1, 40, 30)
+    sc = O.OracleScene(meshes, sky)
+    film, st = sc.render(cam, 2, 0, 3, stats=True)
+    assert st["samples"] == 2 * 40 * 30 and st["rays"] == st["samples"]
+    cd = camera_desc(cam)
+    rays = np.zeros((40 * 30, 6), np.float32)
+    for y in range(30):
+        for x in range(40):
+            O.lib().orc_camera_ray(C.byref(cd), C.c_float(x + 0.5), C.c_float(y + 0.5), rays[y * 40 + x].ctypes.data)
+    ref = sc.intersect(rays)
+    assert st["boxes"] == 2 * int(ref["bb_hits"].sum() + ref["bb_misses"].sum())
+    assert st["tris"] == 2 * int(ref["tri_isects"].sum())
+    assert np.array_equal(film, sc.render(cam, 2, 0, 3))
+    # deeper paths trace more rays, never more than depth + 1 per sample
+    _, st5 = sc.render(cam, 2, 5, 3, stats=True)
+    assert st["rays"] < st5["rays"] <= 6 * st5["samples"]
