@@ -26,6 +26,9 @@
 #include <vector>
 #include <atomic>
 
+#include <pthread.h>
+#include <sched.h>
+
 #include "common/utilities/Common.h"
 #include "common/parsers/pbrt_parser.h"
 #include "cpu/acceleration/bvh.h"
@@ -260,6 +263,25 @@ void tex_probe(poly::Scene* scene, const std::vector<float>& rays, const std::st
    write_file(out_path, out.data(), out.size() * sizeof(TexProbe));
 }
 
+// Pins worker t to one core, as src/cpu/main.cpp:197-205 pins thread i to
+// CPU i; here the t-th CPU of the process's own affinity set, so that the
+// pinning also holds where the process may use only some of the host's CPUs.
+static void pin_thread(std::thread& th, unsigned t) {
+   cpu_set_t allowed;
+   CPU_ZERO(&allowed);
+   if (sched_getaffinity(0, sizeof(allowed), &allowed) != 0) return;
+   const int n = CPU_COUNT(&allowed);
+   if (n <= 0) return;
+   int want = (int)(t % (unsigned)n), cpu = -1;
+   for (int c = 0; c < CPU_SETSIZE; c++)
+      if (CPU_ISSET(c, &allowed) && want-- == 0) { cpu = c; break; }
+   if (cpu < 0) return;
+   cpu_set_t one;
+   CPU_ZERO(&one);
+   CPU_SET(cpu, &one);
+   (void)pthread_setaffinity_np(th.native_handle(), sizeof(one), &one);
+}
+
 void usage() {
    std::fprintf(stderr,
       "usage: ref_driver (--ptxm scene.bin | --pbrt scene.pbrt) [--nodes out] [--pairs out] [--describe out.json]\n"
@@ -399,6 +421,7 @@ int main(int argc, char** argv) {
                }
                checksum += local;
             });
+            pin_thread(pool.back(), t);
          }
          for (auto& th : pool) th.join();
       }
@@ -414,7 +437,10 @@ int main(int argc, char** argv) {
       // spawned as src/cpu/main.cpp:187-211 does.
       const auto s = std::chrono::steady_clock::now();
       std::vector<std::thread> pool;
-      for (unsigned t = 0; t < threads; t++) pool.push_back(runner->Spawn((int)t));
+      for (unsigned t = 0; t < threads; t++) {
+         pool.push_back(runner->Spawn((int)t));
+         pin_thread(pool.back(), t);
+      }
       for (auto& th : pool) th.join();
       const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - s).count();
       const double samples = double(runner->Bounds.x) * runner->Bounds.y * runner->NumSamples;
