@@ -134,5 +134,76 @@ def save_tex_probe(name):
     np.savez_compressed(OUT / f"{name}.npz", rays=rays, probe=probe.view(np.uint32).reshape(len(rays), -1))
     print(f"wrote {name}.npz: {len(rays)} rays, {int((probe['flags'] & 8).astype(bool).sum())} texel lookups")
 
+def save_glossy_draws(name="glossy_draws"):
+    """The reference's GlossyBRDF::sample on 300 incoming directions and
+    roughnesses, with the uniforms its generator handed out (the reference
+    driver's --glossy-cases): the inputs and the reference's rows."""
+    import struct
+    from tests.test_glossy import _cases
+    cases = _cases(300, 11)
+    with tempfile.TemporaryDirectory() as d:
+        with open(f"{d}/in.bin", "wb") as fh:
+            fh.write(struct.pack("<I", len(cases)))
+            for v, r, sd in cases:
+                fh.write(struct.pack("<4fI", float(v[0]), float(v[1]), float(v[2]), float(r), sd))
+        O.run_reference(["--glossy-cases", "in.bin", "out.bin"], cwd=d)
+        rows = np.fromfile(f"{d}/out.bin", np.float32).reshape(len(cases), 132)
+    np.savez_compressed(OUT / f"{name}.npz", incoming=np.array([c[0] for c in cases], np.float32),
+                        roughness=np.array([c[1] for c in cases], np.float32),
+                        seed=np.array([c[2] for c in cases], np.uint32), rows=rows)
+    print(f"wrote {name}.npz: {len(cases)} cases")
+
+
+# Scenes whose reference BVH is pinned by hash (tests/test_abi_and_builder.py):
+# name -> generator of the meshes.
+LARGE_BVH_SCENES = {
+    "closest_hit_subdiv8": lambda: S.closest_hit_workload(8, 1024, seed=2)[0],
+    "stress": lambda: S.stress_workload(1 << 10)[0],
+    "batched_render": lambda: S.batched_render_workload(8, 32, 16, 64, 8)[0],
+}
+
+
+def save_bvh_hashes(names=tuple(LARGE_BVH_SCENES), name="bvh_hashes"):
+    """SHA-256 of the reference's own node and leaf-pair arrays (bvh::bound +
+    bvh::compact, 32-byte nodes and (mesh, face) pairs as the driver writes
+    them) for the large workload scenes: 1.31M, 10.5M and 43.25M faces."""
+    import hashlib
+    import json
+    import time
+    path = OUT / f"{name}.json"
+    table = json.loads(path.read_text()) if path.exists() else {}
+    for key in names:
+        meshes = LARGE_BVH_SCENES[key]()
+        with tempfile.TemporaryDirectory() as d:
+            S.write_ptxm(f"{d}/s.bin", meshes)
+            t0 = time.time()
+            O.run_reference(["--ptxm", f"{d}/s.bin", "--nodes", f"{d}/n.bin", "--pairs", f"{d}/p.bin"], cwd=d,
+                            timeout=3600)
+            dt = time.time() - t0
+            nodes = np.fromfile(f"{d}/n.bin", np.uint8)
+            pairs = np.fromfile(f"{d}/p.bin", np.uint8)
+        table[key] = {"faces": int(sum(m.num_faces for m in meshes)), "nodes": int(nodes.size // 32),
+                      "pairs": int(pairs.size // 8),
+                      "sha256_nodes": hashlib.sha256(nodes.tobytes()).hexdigest(),
+                      "sha256_pairs": hashlib.sha256(pairs.tobytes()).hexdigest(),
+                      "reference_seconds": round(dt, 1)}
+        del meshes, nodes, pairs
+        path.write_text(json.dumps(table, indent=1) + "\n")
+        print(f"{key}: {table[key]}", flush=True)
+
+
 if __name__ == "__main__":
-    main()
+    if O.build_reference() is None:
+        raise SystemExit("reference tree not found; fixtures are only regenerated where /root/reference exists")
+    what = sys.argv[1:] or ["all"]
+    if "all" in what:
+        main()
+        save_glossy_draws()
+        save_bvh_hashes()
+    if "glossy" in what:
+        save_glossy_draws()
+    if "bvh_hashes" in what:
+        save_bvh_hashes()
+    for key in what:
+        if key in LARGE_BVH_SCENES:
+            save_bvh_hashes((key,))

@@ -92,3 +92,40 @@ def test_builder_matches_oracle(case, threads):
     on, op = O.OracleScene(meshes).bvh()
     assert nodes.tobytes() == on.tobytes()
     assert np.array_equal(pairs, op)
+
+
+# Large workload scenes: the SHA-256 of the reference's own node and leaf-pair
+# arrays for them (tests/golden/bvh_hashes.json, written by make_golden.py from
+# the reference compiled from its own sources) against the library's builder
+# at one thread and at every core, and the oracle's restated builder.
+def _large_cases():
+    import json
+    table = json.loads((F.GOLDEN / "bvh_hashes.json").read_text())
+    gens = {"closest_hit_subdiv8": lambda: S.closest_hit_workload(8, 1024, seed=2)[0],
+            "stress": lambda: S.stress_workload(1 << 10)[0],
+            "batched_render": lambda: S.batched_render_workload(8, 32, 16, 64, 8)[0]}
+    return [(k, gens[k], table[k]) for k in gens if k in table]
+
+
+def _sha(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", _large_cases(), ids=lambda c: c[0])
+def test_builder_matches_reference_hashes_at_scale(case):
+    import os
+    name, gen, ref = case
+    meshes = gen()
+    assert sum(m.num_faces for m in meshes) == ref["faces"]
+    all_cores = os.cpu_count() or 1
+    threads = [all_cores] if name == "batched_render" else [1, all_cores]   # 43M faces: one build
+    for t in threads:
+        nodes, pairs = build_bvh(meshes, threads=t)
+        assert (len(nodes), len(pairs)) == (ref["nodes"], ref["pairs"]), t
+        assert _sha(nodes) == ref["sha256_nodes"], t
+        assert _sha(pairs) == ref["sha256_pairs"], t
+        del nodes, pairs
+    if name != "batched_render":
+        on, op = O.OracleScene(meshes).bvh()
+        assert _sha(on) == ref["sha256_nodes"] and _sha(op) == ref["sha256_pairs"]

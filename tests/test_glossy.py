@@ -7,6 +7,7 @@ project-owned acos that CPU and GPU use instead, against the exact function;
 and closed forms (roughness 0 gives the mirror direction, the reflectance mix)."""
 import math
 import struct
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -34,15 +35,10 @@ def _cases(n, seed):
     return out
 
 
-@needs_ref
-def test_glossy_lobe_matches_reference_on_its_own_draws(tmp_path):
-    cases = _cases(300, 11)
-    with open(tmp_path / "in.bin", "wb") as fh:
-        fh.write(struct.pack("<I", len(cases)))
-        for v, r, s in cases:
-            fh.write(struct.pack("<4fI", float(v[0]), float(v[1]), float(v[2]), float(r), s))
-    O.run_reference(["--glossy-cases", "in.bin", "out.bin"], cwd=str(tmp_path))
-    res = np.fromfile(tmp_path / "out.bin", np.float32).reshape(len(cases), 132)
+GOLDEN = Path(__file__).resolve().parent / "golden" / "glossy_draws.npz"
+
+
+def _check_against_reference_rows(cases, res):
     multi = 0
     for (v, r, _), row in zip(cases, res):
         n = int(row[3:4].view(np.uint32)[0])
@@ -52,6 +48,27 @@ def test_glossy_lobe_matches_reference_on_its_own_draws(tmp_path):
         assert np.array_equal(got.view(np.uint32), row[:3].view(np.uint32)), (v, r)
         multi += used > 1
     assert multi > 10                                          # rejections were exercised
+
+
+@needs_ref
+def test_glossy_lobe_matches_reference_on_its_own_draws(tmp_path):
+    cases = _cases(300, 11)
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(struct.pack("<I", len(cases)))
+        for v, r, s in cases:
+            fh.write(struct.pack("<4fI", float(v[0]), float(v[1]), float(v[2]), float(r), s))
+    O.run_reference(["--glossy-cases", "in.bin", "out.bin"], cwd=str(tmp_path))
+    res = np.fromfile(tmp_path / "out.bin", np.float32).reshape(len(cases), 132)
+    _check_against_reference_rows(cases, res)
+
+
+def test_glossy_lobe_matches_reference_golden_draws():
+    # the reference's rows recorded by tests/golden/make_golden.py (no reference build needed)
+    with np.load(GOLDEN, allow_pickle=False) as z:
+        inc, rough, seeds, rows = z["incoming"], z["roughness"], z["seed"], z["rows"]
+    cases = _cases(300, 11)
+    assert np.array_equal(inc, np.array([c[0] for c in cases])) and np.array_equal(seeds, [c[2] for c in cases])
+    _check_against_reference_rows(list(zip(inc, rough, seeds)), rows)
 
 
 def test_project_acos_accuracy():
