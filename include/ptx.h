@@ -254,7 +254,10 @@ int ptx_film_write_pfm(const float *film_rgb, uint32_t width, uint32_t height, c
 /* Round form of ptx_render_batch and the server (ptx_tuning.render_mode). */
 enum { PTX_RENDER_AUTO = 0, PTX_RENDER_REGEN = 1, PTX_RENDER_WAVEFRONT = 2 };
 
-/* Tuning knobs shared by ptx_render_batch and the server; zero = default. */
+/* ptx_tuning.plain_primary */
+enum { PTX_PLAIN_DEFAULT = 0, PTX_PLAIN_CACHED = 1, PTX_PLAIN_RETRACE = 2 };
+
+/* Tuning knobs shared by ptx_render, ptx_render_batch and the server; zero = default. */
 typedef struct ptx_tuning {
    uint32_t paths_per_wave;  /* path slots per round                                  */
    uint32_t block_size;      /* threads per block of the traversal kernel (256)       */
@@ -268,6 +271,10 @@ typedef struct ptx_tuning {
    uint32_t leaf_lanes;      /* traversal: lanes of a wave that must reach a leaf     */
                              /* before the wave runs a leaf step (1..64; 0 = 1 for   */
                              /* ray queries, 16 for rendering)                       */
+   uint32_t plain_primary;   /* ptx_render, PTX_PLAIN_*: CACHED (default) traces each */
+                             /* pixel's primary ray once and starts every sample     */
+                             /* from it; RETRACE traces it again for every sample,   */
+                             /* as the reference's CUDA path does.  Same films.      */
 } ptx_tuning;
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);

@@ -152,12 +152,14 @@ class Device:
             self.handle = None
 
     def set_tuning(self, paths_per_wave: int = 0, block_size: int = 0, lds_stack: int = 0, refill_lanes: int = 0,
-                   blocks_per_cu: int = 0, render_mode: int = 0, shade_lanes: int = 0, leaf_lanes: int = 0) -> None:
+                   blocks_per_cu: int = 0, render_mode: int = 0, shade_lanes: int = 0, leaf_lanes: int = 0,
+                   plain_primary: int = 0) -> None:
         """render_mode: 0 auto, 1 path regeneration, 2 wavefront queues;
         shade_lanes: lanes a regeneration wave gathers before it shades;
-        leaf_lanes: lanes a traversal wave gathers at leaves before a leaf step."""
+        leaf_lanes: lanes a traversal wave gathers at leaves before a leaf step;
+        plain_primary: render(), 0/1 primary rays traced once per pixel, 2 traced again every sample."""
         t = _abi.Tuning(paths_per_wave, block_size, lds_stack, refill_lanes, blocks_per_cu, render_mode, shade_lanes,
-                        leaf_lanes)
+                        leaf_lanes, plain_primary)
         check(load().ptx_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self) -> dict:

@@ -449,6 +449,7 @@ int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
    if (t->render_mode > PTX_RENDER_WAVEFRONT) return set_error(PTX_ERR_INVALID, "render_mode must be 0, 1 or 2");
    if (t->shade_lanes > 64) return set_error(PTX_ERR_INVALID, "shade_lanes must be 0..64");
    if (t->leaf_lanes > 64) return set_error(PTX_ERR_INVALID, "leaf_lanes must be 0..64");
+   if (t->plain_primary > PTX_PLAIN_RETRACE) return set_error(PTX_ERR_INVALID, "plain_primary must be 0, 1 or 2");
    set_tuning(dev, *t);
    return PTX_OK;
 }

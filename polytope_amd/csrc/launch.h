@@ -64,8 +64,9 @@ uint32_t paths_max_scenes();
 void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs, const DevScene *scenes,
                   uint32_t nscenes, float *res, uint32_t *work, uint4 *spill, uint32_t max_stack, uint32_t shade_min,
                   uint32_t leaf_min, float4 *nee_scratch, bool glossy, bool textured, uint32_t grid, hipStream_t st);
-// textured: the scene has textured meshes.
-void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, bool textured, uint32_t grid,
-                 hipStream_t st);
+// textured: the scene has textured meshes; cached: start every path from the
+// pixel's primary record in J.primary (written by launch_primary).
+void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, bool textured, bool cached,
+                 uint32_t grid, hipStream_t st);
 
 }  // namespace ptx

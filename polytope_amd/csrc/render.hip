@@ -782,8 +782,11 @@ __global__ void k_zero_floats(float *a, size_t n) {
 // ---------------------------------------------------------------- plain path
 
 // One sample of every pixel: one thread per pixel path, all bounces inline.
-// kTex: the scene has textured meshes.
-template <int kLds, bool kNee, bool kTex>
+// kTex: the scene has textured meshes.  kCached: the path starts from the
+// pixel's primary record (k_primary: CenterSampler gives every sample of a
+// pixel the same primary ray) and traces from bounce 1; otherwise it traces
+// its primary ray itself, as the reference's CUDA path does for every sample.
+template <int kLds, bool kNee, bool kTex, bool kCached>
 __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sample, uint4 *spill) {
    __shared__ uint32_t lds_ref[kLds * 256];
    __shared__ float lds_near[kLds * 256];
@@ -795,12 +798,54 @@ __global__ void __launch_bounds__(256) k_mega(DevScene S, JobDev J, uint32_t sam
       const uint32_t x = p % J.width, y = p / J.width;
       const bool dbg = x == 128 && y == 128;
       float o[3], d[3];
-      camera_ray(J, x, y, o, d);
       float src[3] = {1.f, 1.f, 1.f};
       Radiance L = {0.f, 0.f, 0.f};
       const bool nee = kNee && (J.flags & PTX_JOB_NEE) != 0;
       float direct[3] = {0.f, 0.f, 0.f};
-      for (uint32_t b = 0;; b++) {
+      uint32_t b0 = 0;
+      if constexpr (kCached) {
+         // bounce 0 from the primary record: terminal radiance, or the hit frame
+         const float4 *pr = reinterpret_cast<const float4 *>(J.primary + 16 * (size_t)p);
+         const float4 a = pr[0], bq = pr[1], cq = pr[2], dq = pr[3];
+         const uint32_t w0 = __float_as_uint(a.x);
+         if (w0 & 1u) {
+            L = Radiance{a.y, a.z, a.w};
+            b0 = kNone;
+         } else {
+            MeshRec M;
+            if (w0 & 2u) {                      // textured Lambert: the texel rides in the record
+               M.brdf = PTX_BRDF_LAMBERT;
+               texel_refl(w0 >> 8, M.refl);
+            } else {
+               M = S.meshes[w0 >> 2];
+            }
+            Frame fr;
+            fr.lx = a.y; fr.ly = a.z; fr.lz = a.w;
+            fr.t1x = bq.x; fr.t1y = bq.y; fr.t1z = bq.z;
+            fr.nx = bq.w; fr.ny = cq.x; fr.nz = cq.y;
+            fr.t2x = cq.z; fr.t2y = cq.w; fr.t2z = dq.x;
+            const float lin[3] = {dq.y, dq.z, dq.w};
+            if (nee && M.brdf == PTX_BRDF_LAMBERT) {
+               for (uint32_t j = 0; j < S.num_lights; j++) {
+                  const float *light = S.lights + 6 * (size_t)j;
+                  float l[3], dist, g;
+                  if (!light_setup(light, fr.lx, fr.ly, fr.lz, fr.nx, fr.ny, fr.nz, l, dist, g)) continue;
+                  if (occluded<kLds>(S, st, fr.lx, fr.ly, fr.lz, l, dist)) continue;
+                  direct[0] = direct[0] + (src[0] * M.refl[0]) * (light[3] * g);
+                  direct[1] = direct[1] + (src[1] * M.refl[1]) * (light[4] * g);
+                  direct[2] = direct[2] + (src[2] * M.refl[2]) * (light[5] * g);
+               }
+            }
+            float wout[3];
+            bounce(M, fr, lin, p, sample, 0u, J.seed, wout, src);
+            o[0] = fr.lx; o[1] = fr.ly; o[2] = fr.lz;
+            d[0] = wout[0]; d[1] = wout[1]; d[2] = wout[2];
+            b0 = 1;
+         }
+      } else {
+         camera_ray(J, x, y, o, d);
+      }
+      for (uint32_t b = b0; b != kNone; b++) {
          if (src_is_zero(src)) break;
          Trav s;
          make_ray(s.r, o[0], o[1], o[2], d[0], d[1], d[2]);
@@ -1301,8 +1346,8 @@ int render_blocks_per_cu(int which) {
    hipError_t e = hipSuccess;
    if (which == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_extend<kRenderLds>, 256, 0);
    else if (which == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_primary<kRenderLds>, 256, 0);
-   else if (which == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, false, false>, 256, 0);
-   else if (which == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, true, false>, 256, 0);
+   else if (which == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, false, false, false>, 256, 0);
+   else if (which == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_mega<kRenderLds, true, false, false>, 256, 0);
    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_connect<kRenderLds>, 256, 0);
    return e == hipSuccess && b > 0 ? b : 1;
 }
@@ -1403,19 +1448,24 @@ void launch_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const 
 }
 
 namespace {
-template <bool T>
+template <bool T, bool C>
 void launch_mega_t(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, uint32_t grid, hipStream_t st) {
    if (J.flags & PTX_JOB_NEE)
-      hipLaunchKernelGGL((dev::k_mega<kRenderLds, true, T>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+      hipLaunchKernelGGL((dev::k_mega<kRenderLds, true, T, C>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
    else
-      hipLaunchKernelGGL((dev::k_mega<kRenderLds, false, T>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
+      hipLaunchKernelGGL((dev::k_mega<kRenderLds, false, T, C>), dim3(grid), dim3(256), 0, st, S, J, sample, spill);
 }
 }  // namespace
 
-void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, bool textured, uint32_t grid,
-                 hipStream_t st) {
-   if (textured) launch_mega_t<true>(S, J, sample, spill, grid, st);
-   else launch_mega_t<false>(S, J, sample, spill, grid, st);
+void launch_mega(const DevScene &S, const JobDev &J, uint32_t sample, uint4 *spill, bool textured, bool cached,
+                 uint32_t grid, hipStream_t st) {
+   if (cached) {
+      if (textured) launch_mega_t<true, true>(S, J, sample, spill, grid, st);
+      else launch_mega_t<false, true>(S, J, sample, spill, grid, st);
+   } else {
+      if (textured) launch_mega_t<true, false>(S, J, sample, spill, grid, st);
+      else launch_mega_t<false, false>(S, J, sample, spill, grid, st);
+   }
 }
 
 }  // namespace ptx

@@ -409,18 +409,32 @@ int RenderEngine::run_plain(const ptx_scene *scene, const ptx_job_desc &desc, fl
    int rc = ensure(cap ? cap : 1u, scene->d.max_stack, 1, 1, 1);
    if (rc) return rc;
    hipStream_t st = stream;
+   // the default traces each pixel's primary ray once (every sample of a pixel
+   // has the same one) and starts every sample from that record
+   const bool cached = tun.plain_primary != PTX_PLAIN_RETRACE;
    float *film = nullptr;
+   uint32_t *primary = nullptr;
    if ((rc = dalloc(&film, (size_t)npix * 3))) return rc;
+   if (cached && (rc = dalloc(&primary, (size_t)npix * 16))) {
+      dfree(film);
+      return rc;
+   }
    JobDev J = job_dev(desc, 0);
    J.film = film;
+   J.primary = primary;
    launch_zero_floats(film, (size_t)npix * 3, st);
+   if (cached)
+      launch_primary(scene->d, J, words + kWork, spill, leaf_lanes_of(tun, true),
+                     std::min<uint32_t>(ext_grid, (npix + 255) / 256), st);
    const uint32_t grid =
          std::min<uint32_t>((desc.flags & PTX_JOB_NEE) ? mega_nee_grid : mega_grid, (npix + 255) / 256);
-   for (uint32_t s = 0; s < desc.spp; s++) launch_mega(scene->d, J, s, spill, scene->has_texture, grid, st);
+   for (uint32_t s = 0; s < desc.spp; s++) launch_mega(scene->d, J, s, spill, scene->has_texture, cached, grid, st);
    launch_finalize(film, npix, desc.spp, st);
    hipError_t e = hipMemcpyAsync(film_host, film, (size_t)npix * 3 * sizeof(float), hipMemcpyDeviceToHost, st);
    if (e == hipSuccess) e = hipStreamSynchronize(st);
+   if (e != hipSuccess) (void)hipGetLastError();
    dfree(film);
+   dfree(primary);
    if (e != hipSuccess) return set_error(PTX_ERR_HIP, std::string("render: ") + hipGetErrorString(e));
    return PTX_OK;
 }

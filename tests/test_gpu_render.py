@@ -173,6 +173,24 @@ def test_next_event_estimation_matches_oracle(dev, depth):
         assert (ref >= off).all() and (ref > off).any()
 
 
+@pytest.mark.parametrize("nee", [False, True])
+def test_plain_path_retracing_primaries_equals_cached(dev, nee):
+    # ptx_render traces each pixel's primary ray once by default; tracing it
+    # again for every sample (the reference CUDA path's shape) gives the same film
+    meshes, lights, cam, sky = _nee_scene()
+    sc = Scene(dev, meshes, sky, lights=lights)
+    print(f"case plain retrace nee={nee}", flush=True)
+    ref = O.OracleScene(meshes, sky, lights=lights).render(cam, 3, 5, seed=31, nee=nee)
+    cached = render(sc, cam, 3, 5, 31, nee=nee)
+    dev.set_tuning(plain_primary=_abi.PLAIN_RETRACE)
+    try:
+        retraced = render(sc, cam, 3, 5, 31, nee=nee)
+    finally:
+        dev.set_tuning()
+    assert np.array_equal(cached.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(retraced.view(np.uint32), ref.view(np.uint32))
+
+
 def test_next_event_jobs_mixed_with_plain_jobs(dev):
     # one round holding a next-event job and an ordinary job of the same scene:
     # the ordinary job's film is the one it has without next-event jobs around
