@@ -793,19 +793,34 @@ def stress_workload(n_rays: int = 1 << 24, spheres: int = 32, n_lat: int = 256, 
     return meshes, np.concatenate([feat, inc], 0), n_feat, t_target.astype(np.float32)
 
 
-def serving_requests(n: int, rate_hz: float, seed: int = 4, subdivs=(5, 6, 7, 8)):
+# Scenes of the serving stream: the icosphere subdivisions of each scene's
+# spheres, from 10,242 faces (two spheres of 5,120 and the ground) to
+# 1,966,082 (one sphere of 1,310,720 and two of 327,680).
+SERVING_SCENES = ((4, 4), (5,), (6,), (7,), (8,), (8, 7, 7))
+
+
+def serving_scene(spec, index: int, seed: int = 4):
+    """One scene of the serving stream: displaced icospheres of subdivisions
+    ``spec`` side by side (radius 1 for the first, 0.5 for the others) above a
+    ground quad (mirror for odd ``index``), with a constant sky."""
+    meshes = []
+    for k, sd in enumerate(spec):
+        center = (0.0, 0.0, 0.0) if k == 0 else (-1.6 + 3.2 * (k - 1), -0.5, 0.8)
+        v, f = icosphere(sd, radius=1.0 if k == 0 else 0.5, center=center, noise=0.05, seed=seed + 7 * index + k)
+        meshes.append(MeshData(v, f, refl=(0.7, 0.6, 0.5) if k == 0 else (0.5, 0.6, 0.7),
+                               brdf=LAMBERT if k != 1 else MIRROR))
+    gv, gf = quad(6.0, -1.05)
+    meshes.append(MeshData(gv, gf, brdf=MIRROR if index % 2 else LAMBERT, refl=(0.6, 0.6, 0.6)))
+    return meshes, (140.0, 160.0, 190.0)
+
+
+def serving_requests(n: int, rate_hz: float, seed: int = 4, scenes=SERVING_SCENES):
     """Poisson request stream for the render service: ``n`` jobs at rate ``rate_hz``
-    over displaced-icosphere scenes (subdivisions ``subdivs``: 20k to 1.3M faces),
+    over the displaced-icosphere scenes ``scenes`` (10k to 2M faces by default),
     resolution between 256x256 and 1920x1080 (log-uniform area), spp log-uniform in
     [4, 256], depth uniform in 4..12.  Returns (scenes, [(arrival_s, scene_index,
     camera, spp, depth, seed)])."""
-    scenes_ = []
-    for i, sd in enumerate(subdivs):
-        v, f = icosphere(sd, radius=1.0, noise=0.05, seed=seed + i)
-        gv, gf = quad(6.0, -1.05)
-        scenes_.append(([MeshData(v, f, refl=(0.7, 0.6, 0.5)),
-                         MeshData(gv, gf, brdf=MIRROR if i % 2 else LAMBERT, refl=(0.6, 0.6, 0.6))],
-                        (140.0, 160.0, 190.0)))
+    scenes_ = [serving_scene(spec, i, seed) for i, spec in enumerate(scenes)]
     rng = Rng(seed)
     u = [rng.uniform(n) for _ in range(6)]
     gaps = -np.log(np.maximum(u[0], 1e-12)) / rate_hz      # exponential inter-arrival times
@@ -817,7 +832,7 @@ def serving_requests(n: int, rate_hz: float, seed: int = 4, subdivs=(5, 6, 7, 8)
         h = int(min(1080, max(256, round(area / w))))
         spp = int(round(math.exp(math.log(4) + (math.log(256) - math.log(4)) * u[2][i])))
         depth = 4 + int(u[3][i] * 9) % 9
-        sc = int(u[4][i] * len(subdivs)) % len(subdivs)
+        sc = int(u[4][i] * len(scenes)) % len(scenes)
         cam = Camera(look_at((0.4, 0.6, -3.2), (0.0, 0.0, 0.0)), 50.0, w, h)
         reqs.append((float(arrivals[i]), sc, cam, spp, depth, seed + 100 + i))
     return scenes_, reqs

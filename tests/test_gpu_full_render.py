@@ -30,17 +30,27 @@ def dev():
 @pytest.mark.timeout(900)
 def test_gpu_full_size_batched_render_rows_match_oracle(dev):
     print("case full-size render: building 43M-face scene", flush=True)
-    meshes, sky, jobs = S.batched_render_workload(8, 32, 16, 8, 8)
-    jobs = [jobs[0], jobs[7], jobs[15]]                 # 256x256, mid size, 1920x1080; 8 spp, depth 8
+    meshes, sky, jobs = S.batched_render_workload(8, 32, 16, 64, 8)
+    # 256x256 at the workload's 64 spp; a mid-size job and 1920x1080 at 8 spp; depth 8
+    jobs = [jobs[0], (jobs[7][0], 8, jobs[7][2], jobs[7][3]), (jobs[15][0], 8, jobs[15][2], jobs[15][3])]
+    assert jobs[0][1] == 64
     sc = Scene(dev, meshes, sky)
     print(f"case full-size render: scene built ({sc.info()['build_seconds']:.1f} s)", flush=True)
     films = render_batch([(sc, cam, spp, depth, seed) for cam, spp, depth, seed in jobs])
     for f in films:
         assert np.isfinite(f).all() and f.min() >= 0 and f.max() <= 255
-    oracle = O.OracleScene(meshes, sky, bvh=sc.bvh())
+    # the oracle builds its own tree (the restated bvh::bound + compact); it is
+    # the library's byte for byte
+    print("case full-size render: oracle tree", flush=True)
+    oracle = O.OracleScene(meshes, sky)
+    on, op = oracle.bvh()
+    ln, lp = sc.bvh()
+    assert np.array_equal(on.view(np.uint8), ln.view(np.uint8)) and np.array_equal(op, lp)
+    del on, op, ln, lp
     for (cam, spp, depth, seed), film in zip(jobs, films):
-        print(f"case full-size render: oracle rows of {cam.width}x{cam.height}", flush=True)
-        ref = oracle.render(cam, spp, depth, seed, row_step=61)
-        rows = np.arange(0, cam.height, 61)
+        step = 16 if spp == 64 else 61
+        print(f"case full-size render: oracle rows of {cam.width}x{cam.height} at {spp} spp", flush=True)
+        ref = oracle.render(cam, spp, depth, seed, row_step=step)
+        rows = np.arange(0, cam.height, step)
         assert np.array_equal(film[rows].view(np.uint32), ref[rows].view(np.uint32))
     sc.close()

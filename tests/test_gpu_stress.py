@@ -59,7 +59,13 @@ def test_gpu_stress_watertight_full_size(dev):
     # parity on a sample: every 4099th ray plus the first 2048 feature rays
     idx = np.unique(np.concatenate([np.arange(0, len(rays), 4099), np.arange(2048),
                                     n_feat // 2 + np.arange(2048)]))
-    ref = O.OracleScene(meshes, bvh=sc.bvh()).intersect(rays[idx])
+    # the oracle's own tree (restated bvh::bound + compact) is the library's byte for byte
+    oracle = O.OracleScene(meshes)
+    on, op = oracle.bvh()
+    ln, lp = sc.bvh()
+    assert np.array_equal(on.view(np.uint8), ln.view(np.uint8)) and np.array_equal(op, lp)
+    del on, op, ln, lp
+    ref = oracle.intersect(rays[idx])
     rh = ref["hit"] == 1
     assert np.array_equal(hit[idx], rh)
     assert np.array_equal(h["mesh"][idx][rh], ref["mesh"][rh])
@@ -81,6 +87,6 @@ def test_gpu_stress_render_depth16(dev):
     sc = Scene(dev, meshes, sky)
     print("case stress render depth 16", flush=True)
     (film,) = render_batch([(sc, cam, 2, 16, 77)])
-    ref = O.OracleScene(meshes, sky, bvh=sc.bvh()).render(cam, 2, 16, seed=77)
+    ref = O.OracleScene(meshes, sky).render(cam, 2, 16, seed=77)      # the oracle's own tree
     assert np.array_equal(film.view(np.uint32), ref.view(np.uint32))
     sc.close()
