@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--serving-loaded-requests", type=int, default=64,
                     help="second stream at a rate near the GPU's capacity")
     ap.add_argument("--serving-loaded-rate", type=float, default=64.0)
+    ap.add_argument("--skip-accounting", action="store_true",
+                    help="skip the render workload's ray and byte accounting (oracle on a row sample)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
@@ -74,6 +76,18 @@ def algorithmic_bytes(counters: np.ndarray) -> np.ndarray:
     reference pops and box-tests (num_bb_hits + num_bb_misses) and N_tri its face tests."""
     n_box = counters["bb_hits"].astype(np.float64) + counters["bb_misses"]
     return 24.0 + 20.0 + 32.0 * n_box + 36.0 * counters["tri_isects"].astype(np.float64)
+
+
+def host_threads(args) -> int:
+    """Threads for the CPU baselines: the cores this process may run on (nproc),
+    or fewer when OMP_NUM_THREADS declares a smaller share of the host."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
 def cpu_model() -> str:
@@ -122,6 +136,39 @@ def cpu_baseline(meshes, rays: np.ndarray, threads: int, repeat: int):
             "cpu_model": cpu_model()}
 
 
+def render_accounting(orc, jobs, t_batched: float, threads: int, row_step: int = 64):
+    """Rays and algorithmic bytes of the batched render: the oracle's
+    restatement of the integrator on every row_step-th row of every job counts
+    the rays a sample traces and the reference's per-ray counters over them,
+    scaled to the whole films.  The batched path traces a pixel's primary ray
+    once (not once per sample), so it traces (spp - 1) * pixels fewer rays than
+    the integrator.  Bytes: per traced ray B_ray = 24 + 20 + 32 * N_box + 36 *
+    N_tri, per sample 64 B of primary record and 12 B of result."""
+    tot = {"rays": 0.0, "boxes": 0.0, "tris": 0.0, "samples": 0.0}
+    gpu_rays = 0.0
+    t0 = time.perf_counter()
+    for cam, spp, depth, seed in jobs:
+        _, st = orc.render(cam, spp, depth, seed, threads=threads, row_step=row_step, stats=True)
+        rows = len(range(0, cam.height, row_step))
+        scale = cam.height / rows
+        for k in tot:
+            tot[k] += st[k] * scale
+        gpu_rays += (st["rays"] - (spp - 1) * rows * cam.width) * scale
+    dt = time.perf_counter() - t0
+    rays = tot["rays"]
+    mean_box, mean_tri = tot["boxes"] / rays, tot["tris"] / rays
+    b_ray = 24 + 20 + 32 * mean_box + 36 * mean_tri
+    total_bytes = gpu_rays * b_ray + tot["samples"] * (64 + 12)
+    return {"method": f"oracle integrator on every {row_step}th row of every job ({dt:.1f} s), scaled to the films",
+            "samples": round(tot["samples"]), "integrator_rays": round(rays),
+            "rays_per_sample": round(rays / tot["samples"], 4), "extend_rays": round(gpu_rays),
+            "extend_mrays_per_s": round(gpu_rays / t_batched / 1e6, 1),
+            "mean_box_tests_per_ray": round(mean_box, 2), "mean_face_tests_per_ray": round(mean_tri, 2),
+            "bytes_per_ray": round(b_ray, 1), "algorithmic_bytes": round(total_bytes),
+            "achieved_gbs": round(total_bytes / t_batched / 1e9, 1),
+            "frac_of_hbm_peak": round(total_bytes / t_batched / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def render_section(dev, args):
     """Plain (one job per launch series) vs batched (all jobs' samples in shared
     rounds: path regeneration, and the wavefront form for comparison) samples/s
@@ -151,12 +198,25 @@ def render_section(dev, args):
     t_wave = time.perf_counter() - tw
     dev.set_tuning()
     print(f"[bench] batched wavefront: {samples / t_wave / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
+    # plain: one job per call, one launch per sample; each pixel's primary ray
+    # traced once (the default), and again for every sample (the reference
+    # CUDA path's shape)
     tp = time.perf_counter()
     films_p = [render(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
     t_plain = time.perf_counter() - tp
     print(f"[bench] plain: {samples / t_plain / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
+    dev.set_tuning(plain_primary=_abi.PLAIN_RETRACE)
+    tu = time.perf_counter()
+    films_u = [render(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
+    t_uncached = time.perf_counter() - tu
+    dev.set_tuning()
+    print(f"[bench] plain, primary rays traced every sample: {samples / t_uncached / 1e6:.1f} Msamples/s",
+          file=sys.stderr, flush=True)
     same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) and
-               np.array_equal(a.view(np.uint32), w.view(np.uint32)) for a, b, w in zip(films_b, films_p, films_w))
+               np.array_equal(a.view(np.uint32), w.view(np.uint32)) and
+               np.array_equal(a.view(np.uint32), u.view(np.uint32))
+               for a, b, w, u in zip(films_b, films_p, films_w, films_u))
+    del films_u
     # the same geometry and BVH with textured Lambert and glossy meshes
     bvh = scene.bvh()
     mat_scene = Scene(dev, S.with_materials(meshes), sky, bvh=bvh)
@@ -174,30 +234,44 @@ def render_section(dev, args):
           f"Msamples/s", file=sys.stderr, flush=True)
     mat_scene.close()
     cpu = None
+    accounting = None
+    orc = None
+    if not (args.skip_cpu and args.skip_accounting):
+        import oracle as O
+        orc = O.OracleScene(meshes, sky, bvh=bvh)
+    del bvh
+    if not args.skip_accounting:
+        accounting = render_accounting(orc, jobs, t_batched, host_threads(args))
+        print(f"[bench] render accounting: {accounting['extend_rays'] / 1e9:.3f} G extend rays, "
+              f"{accounting['extend_mrays_per_s']:.0f} Mrays/s, {accounting['achieved_gbs']:.0f} GB/s of algorithmic "
+              f"bytes", file=sys.stderr, flush=True)
     if not args.skip_cpu:
         # CPU baseline: the oracle (port of the reference's integrator) on the same
         # scene and tree, the whole first job, on the host's cores
-        import oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = host_threads(args)
         cam0, spp0, d0, seed0 = jobs[0]
-        orc = O.OracleScene(meshes, sky, bvh=bvh)
-        del bvh
         tc = time.perf_counter()
         film0 = orc.render(cam0, spp0, d0, seed0, threads=threads)
         t_cpu = time.perf_counter() - tc
         n_cpu = cam0.height * cam0.width * spp0
-        del orc
         cpu = {"value": round(n_cpu / t_cpu, 1), "unit": "samples/s", "cores": threads, "kind": "port",
                "sample": f"job 0 ({cam0.width}x{cam0.height}, {spp0} spp, depth {d0}): {n_cpu} samples in "
                          f"{t_cpu:.2f} s, oracle restatement of PathTraceIntegrator on the same tree",
                "film_equals_gpu": bool(np.array_equal(film0.view(np.uint32), films_b[0].view(np.uint32)))}
         print(f"[bench] render CPU baseline: {n_cpu / t_cpu / 1e6:.3f} Msamples/s", file=sys.stderr, flush=True)
+    del orc
     return {"workload": f"icosphere subdiv {args.render_subdiv} x {args.render_instances + 1} instances + ground "
                         f"({sum(m.num_faces for m in meshes)} tris), 16 jobs 256x256..1920x1080, "
                         f"{args.render_spp} spp, depth 8, 70% Lambert / 30% mirror",
             "samples": samples, "batched_s": round(t_batched, 3), "plain_s": round(t_plain, 3),
             "batched_samples_per_s": round(samples / t_batched, 1), "plain_samples_per_s": round(samples / t_plain, 1),
             "batched_vs_plain": round(t_plain / t_batched, 3),
+            "plain_uncached_samples_per_s": round(samples / t_uncached, 1),
+            "batched_vs_plain_uncached": round(t_uncached / t_batched, 3),
+            "plain_note": "plain = one job per call, one launch per sample, each pixel's primary ray traced once "
+                          "(as the batched path does); plain_uncached traces it again for every sample, as the "
+                          "reference CUDA path does",
+            "accounting": accounting,
             "wavefront_samples_per_s": round(samples / t_wave, 1), "films_identical": same,
             "bvh_build_s": round(info["build_seconds"], 2),
             "materials": {"workload": "the same meshes and BVH, Lambert meshes textured (512x512 RGBA8), mirrors "
@@ -247,38 +321,70 @@ def plain_cpu_render_section(dev, args, threads: int):
     return out
 
 
-def serving_section(dev, args, n_requests: int, rate: float):
-    """Poisson stream of render jobs through the continuous-batching server."""
-    from polytope_amd import Scene, Server
+def _latency_stats(lat, samples: float, span: float) -> dict:
+    lat = np.array(lat)
+    return {"throughput_samples_per_s": round(samples / span, 1),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p95_s": round(float(np.percentile(lat, 95)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4)}
+
+
+def serving_section(dev, args, n_requests: int, rate: float, burst: bool = False):
+    """Poisson stream of render jobs (or, with burst, all of them at once) through
+    the continuous-batching server, and the same stream through the plain path:
+    one job per render() call, in arrival order, as a caller without the server
+    would serve it.  Latency = completion - arrival."""
+    from polytope_amd import Scene, Server, render
     from polytope_amd import scenes as S
     scene_defs, reqs = S.serving_requests(n_requests, rate)
+    if burst:
+        reqs = [(0.0,) + r[1:] for r in reqs]
     scenes_ = [Scene(dev, m, sky) for m, sky in scene_defs]
+    total = sum(cam.width * cam.height * spp for _, _, cam, spp, _, _ in reqs)
+    # warm both paths (kernel loading, engine buffers) outside the timed streams
+    render(scenes_[0], reqs[0][2], 1, 4, 1)
     srv = Server(dev, hbm_budget_bytes=64 << 30)
+    srv.wait(srv.submit(scenes_[0], reqs[0][2], 1, 4, 1))
     ids = []
     t0 = time.perf_counter()
     for arr, si, cam, spp, depth, seed in reqs:
         delay = arr - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
-        ids.append((srv.submit(scenes_[si], cam, spp, depth, seed), cam.width * cam.height * spp))
-    lat, samples, first, last = [], 0, None, None
-    for jid, n in ids:
+        ids.append(srv.submit(scenes_[si], cam, spp, depth, seed))
+    # the server's times are seconds on its own clock; a job is submitted at its arrival
+    lat, first, last = [], None, None
+    for jid in ids:
         _, st = srv.wait(jid)
         lat.append(st["end_time"] - st["submit_time"])
-        samples += n
         first = st["submit_time"] if first is None else min(first, st["submit_time"])
         last = st["end_time"] if last is None else max(last, st["end_time"])
     srv.close()
+    out = {"requests": len(reqs), "rate_hz": None if burst else rate, "arrivals": "all at t=0" if burst else "Poisson",
+           "scenes_tris": [sum(m.num_faces for m in d[0]) for d in scene_defs], "samples": total}
+    if not burst:
+        out["offered_samples_per_s"] = round(total / max(1e-9, reqs[-1][0]), 1)
+    out.update(_latency_stats(lat, total, last - first))
+    # the plain path over the same stream
+    lat_p = []
+    t0 = time.perf_counter()
+    for arr, si, cam, spp, depth, seed in reqs:
+        delay = arr - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        render(scenes_[si], cam, spp, depth, seed)
+        lat_p.append(time.perf_counter() - (t0 + arr))
+    span = time.perf_counter() - t0
+    out["plain"] = _latency_stats(lat_p, total, span)
+    out["server_vs_plain_throughput"] = round(out["throughput_samples_per_s"] / out["plain"]["throughput_samples_per_s"], 3)
+    out["server_vs_plain_p50_latency"] = round(out["plain"]["latency_p50_s"] / out["latency_p50_s"], 3)
     for s_ in scenes_:
         s_.close()
-    lat = np.array(lat)
-    offered = sum(cam.width * cam.height * spp for _, _, cam, spp, _, _ in reqs) / max(1e-9, reqs[-1][0])
-    return {"requests": len(reqs), "rate_hz": rate, "offered_samples_per_s": round(offered, 1),
-            "scenes_tris": [sum(m.num_faces for m in d[0]) for d in scene_defs],
-            "throughput_samples_per_s": round(samples / (last - first), 1),
-            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
-            "latency_p95_s": round(float(np.percentile(lat, 95)), 4),
-            "latency_p99_s": round(float(np.percentile(lat, 99)), 4)}
+    print(f"[bench] serving {out['arrivals']} ({len(reqs)} jobs): server {out['throughput_samples_per_s'] / 1e6:.0f} "
+          f"Msamples/s p50 {out['latency_p50_s']:.3f} s p99 {out['latency_p99_s']:.3f} s; plain "
+          f"{out['plain']['throughput_samples_per_s'] / 1e6:.0f} Msamples/s p50 {out['plain']['latency_p50_s']:.3f} s "
+          f"p99 {out['plain']['latency_p99_s']:.3f} s", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -365,7 +471,7 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = host_threads(args)
         sample = rays[n_coh: n_coh + (1 << args.cpu_sample_log2)]
         print(f"[bench] cpu baseline on {len(sample)} rays, {threads} threads", file=sys.stderr, flush=True)
         cpu = cpu_baseline(meshes, sample, threads, args.cpu_repeat)
@@ -400,10 +506,11 @@ def main():
     torch.cuda.empty_cache()
     if not args.skip_render:
         out["render"] = render_section(dev, args)
-        out["render_cpu_runnable"] = plain_cpu_render_section(dev, args, args.cpu_threads or min(16, os.cpu_count() or 1))
+        out["render_cpu_runnable"] = plain_cpu_render_section(dev, args, host_threads(args))
     if not args.skip_serving:
         out["serving"] = serving_section(dev, args, args.serving_requests, args.serving_rate)
         out["serving_loaded"] = serving_section(dev, args, args.serving_loaded_requests, args.serving_loaded_rate)
+        out["serving_burst"] = serving_section(dev, args, args.serving_loaded_requests, 1.0, burst=True)
     print(json.dumps(out), flush=True)
     if args.calibrate_copy_mb:
         src = torch.empty(args.calibrate_copy_mb << 20, dtype=torch.uint8, device="cuda")
