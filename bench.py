@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--leaf-lanes", type=int, default=0,
                     help="lanes a traversal wave gathers at leaves before a leaf step (0 = default)")
+    ap.add_argument("--ray-windows", type=int, default=0,
+                    help="closest-hit record form: 0 default, 1 window records, 2 one node record per step")
     ap.add_argument("--no-coherent", action="store_true", help="skip the coherent-rate measurement")
     ap.add_argument("--skip-render", action="store_true", help="skip the plain vs batched render comparison")
     ap.add_argument("--skip-serving", action="store_true", help="skip the Poisson serving run")
@@ -403,9 +405,9 @@ def main():
     print(f"[bench] workload built: {meshes[0].num_faces} triangles, {n_rays} rays "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     dev = Device(0)
-    if args.lds or args.refill or args.blocks_per_cu or args.leaf_lanes:
+    if args.lds or args.refill or args.blocks_per_cu or args.leaf_lanes or args.ray_windows:
         dev.set_tuning(lds_stack=args.lds, refill_lanes=args.refill, blocks_per_cu=args.blocks_per_cu,
-                       leaf_lanes=args.leaf_lanes)
+                       leaf_lanes=args.leaf_lanes, ray_windows=args.ray_windows)
     sc = Scene(dev, meshes)
     info = sc.info()
     print(f"[bench] scene: {info['num_nodes']} nodes, build {info['build_seconds']:.2f} s, "

@@ -226,6 +226,8 @@ int ptx_intersect_closest_device(ptx_scene *scene, const float *d_rays, uint32_t
  * about 64 levels) is refused with PTX_ERR_INVALID.  Every call on one GPU
  * shares that GPU's array: calls that overlap on one GPU (on different
  * streams, or from different threads) race; serialise them on one stream.
+ * With ray_windows = ON it steps over the scene's window records only when an
+ * earlier ptx_intersect_closest* call built them (building allocates).
  * Asynchronous; results equal ptx_intersect_closest_device's bit for bit. */
 int ptx_intersect_closest_device_noalloc(ptx_scene *scene, const float *d_rays, uint32_t n,
                                          ptx_hit *d_hits, ptx_ray_counters *d_counters,
@@ -257,6 +259,9 @@ enum { PTX_RENDER_AUTO = 0, PTX_RENDER_REGEN = 1, PTX_RENDER_WAVEFRONT = 2 };
 /* ptx_tuning.plain_primary */
 enum { PTX_PLAIN_DEFAULT = 0, PTX_PLAIN_CACHED = 1, PTX_PLAIN_RETRACE = 2 };
 
+/* ptx_tuning.ray_windows */
+enum { PTX_WINDOWS_DEFAULT = 0, PTX_WINDOWS_ON = 1, PTX_WINDOWS_OFF = 2 };
+
 /* Tuning knobs shared by ptx_render, ptx_render_batch and the server; zero = default. */
 typedef struct ptx_tuning {
    uint32_t paths_per_wave;  /* path slots per round                                  */
@@ -275,6 +280,13 @@ typedef struct ptx_tuning {
                              /* pixel's primary ray once and starts every sample     */
                              /* from it; RETRACE traces it again for every sample,   */
                              /* as the reference's CUDA path does.  Same films.      */
+   uint32_t ray_windows;     /* closest-hit queries, PTX_WINDOWS_*: OFF (default)     */
+                             /* steps over one node record at a time; ON steps over  */
+                             /* window records (a node's record and its first-       */
+                             /* visited child's in one 128-byte block, 256 B per     */
+                             /* interior node, built by the scene's first query      */
+                             /* with ON).  Same hits and counters; ON is slower on   */
+                             /* MI355X at every scene size measured.                 */
 } ptx_tuning;
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);

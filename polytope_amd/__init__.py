@@ -153,13 +153,15 @@ class Device:
 
     def set_tuning(self, paths_per_wave: int = 0, block_size: int = 0, lds_stack: int = 0, refill_lanes: int = 0,
                    blocks_per_cu: int = 0, render_mode: int = 0, shade_lanes: int = 0, leaf_lanes: int = 0,
-                   plain_primary: int = 0) -> None:
+                   plain_primary: int = 0, ray_windows: int = 0) -> None:
         """render_mode: 0 auto, 1 path regeneration, 2 wavefront queues;
         shade_lanes: lanes a regeneration wave gathers before it shades;
         leaf_lanes: lanes a traversal wave gathers at leaves before a leaf step;
-        plain_primary: render(), 0/1 primary rays traced once per pixel, 2 traced again every sample."""
+        plain_primary: render(), 0/1 primary rays traced once per pixel, 2 traced again every sample;
+        ray_windows: closest-hit queries, 0/2 one node record per step, 1 window records (node + first-visited
+        child in one 128-byte block; same results, slower on MI355X)."""
         t = _abi.Tuning(paths_per_wave, block_size, lds_stack, refill_lanes, blocks_per_cu, render_mode, shade_lanes,
-                        leaf_lanes, plain_primary)
+                        leaf_lanes, plain_primary, ray_windows)
         check(load().ptx_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self) -> dict:

@@ -86,11 +86,13 @@ JOB_NEE = 1
 class Tuning(C.Structure):
     _fields_ = [("paths_per_wave", C.c_uint32), ("block_size", C.c_uint32), ("lds_stack", C.c_uint32),
                 ("refill_lanes", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("render_mode", C.c_uint32),
-                ("shade_lanes", C.c_uint32), ("leaf_lanes", C.c_uint32), ("plain_primary", C.c_uint32)]
+                ("shade_lanes", C.c_uint32), ("leaf_lanes", C.c_uint32), ("plain_primary", C.c_uint32),
+                ("ray_windows", C.c_uint32)]
 
 
 RENDER_AUTO, RENDER_REGEN, RENDER_WAVEFRONT = 0, 1, 2
 PLAIN_DEFAULT, PLAIN_CACHED, PLAIN_RETRACE = 0, 1, 2
+WINDOWS_DEFAULT, WINDOWS_ON, WINDOWS_OFF = 0, 1, 2
 
 
 class ServerConfig(C.Structure):

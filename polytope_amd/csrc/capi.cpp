@@ -77,8 +77,8 @@ uint32_t leaf_lanes_of(const ptx_tuning &t, bool render) {
    return render ? kDefaultRenderLeafLanes : kDefaultLeafLanes;
 }
 
-uint32_t closest_hit_grid(const ptx_device *dev, const ptx_tuning &t, uint32_t lds, bool extra) {
-   int per_cu = closest_hit_blocks_per_cu(lds, extra);
+uint32_t closest_hit_grid(const ptx_device *dev, const ptx_tuning &t, uint32_t lds, bool extra, bool windows) {
+   int per_cu = closest_hit_blocks_per_cu(lds, extra, windows);
    if (t.blocks_per_cu) per_cu = std::min<int>(per_cu, (int)t.blocks_per_cu);
    return (uint32_t)(dev->num_cus * per_cu);
 }
@@ -98,12 +98,43 @@ void scene_release(ptx_scene *s) {
    }
 }
 
+const Node64 *built_windows(ptx_scene *s, const ptx_tuning &t) {
+   if (t.ray_windows != PTX_WINDOWS_ON) return nullptr;
+   std::lock_guard<std::mutex> lock(s->windows_mu);
+   return s->windows;
+}
+
+const Node64 *scene_windows(ptx_scene *s, const ptx_tuning &t) {
+   if (t.ray_windows != PTX_WINDOWS_ON || s->info.num_interior == 0) return nullptr;
+   std::lock_guard<std::mutex> lock(s->windows_mu);
+   if (s->windows || s->windows_failed) return s->windows;
+   // built on a stream of its own and waited for, so that launches on any
+   // stream see the records
+   const size_t bytes = size_t(4) * s->info.num_interior * sizeof(Node64);
+   void *w = nullptr;
+   hipStream_t st = nullptr;
+   bool ok = hipMalloc(&w, bytes) == hipSuccess;
+   ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+   ok = ok && launch_build_windows(s->d.nodes, s->info.num_interior, static_cast<Node64 *>(w), st) == 0;
+   ok = ok && hipStreamSynchronize(st) == hipSuccess;
+   if (st) (void)hipStreamDestroy(st);
+   if (!ok) {
+      (void)hipGetLastError();
+      if (w) (void)hipFree(w);
+      s->windows_failed = true;
+      return nullptr;
+   }
+   s->windows = static_cast<Node64 *>(w);
+   return s->windows;
+}
+
 // Closest-hit launch with stack-spill scratch taken from the stream-ordered allocator.
 int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
                     ptx_hit_frame *d_frames, hipStream_t stream, const ptx_tuning &t) {
    ptx_device *dev = s->dev;
    const uint32_t lds = lds_depth_of(t);
-   uint32_t grid = closest_hit_grid(dev, t, lds, d_counters != nullptr || d_frames != nullptr);
+   const Node64 *win = n ? scene_windows(s, t) : nullptr;
+   uint32_t grid = closest_hit_grid(dev, t, lds, d_counters != nullptr || d_frames != nullptr, win != nullptr);
    const uint32_t needed_blocks = (n + 255) / 256;
    grid = std::max(1u, std::min(grid, needed_blocks));
    void *scratch = nullptr;
@@ -113,7 +144,7 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
    const uint32_t refill = t.refill_lanes ? std::min(64u, t.refill_lanes) : 1u;
    const uint32_t leaf = leaf_lanes_of(t);
    const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill, leaf,
-                                     stream);
+                                     win, s->root_axis, stream);
    const hipError_t fe = hipFreeAsync(scratch, stream);
    if (rc != 0) return set_error(PTX_ERR_HIP, "closest-hit launch failed");
    if (fe != hipSuccess) return set_error(PTX_ERR_HIP, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
@@ -125,9 +156,11 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
 ptx_device::~ptx_device() = default;
 
 ptx_scene::~ptx_scene() {
-   if (!allocations.empty() && dev) (void)hipSetDevice(dev->ordinal);
+   if ((!allocations.empty() || windows) && dev) (void)hipSetDevice(dev->ordinal);
    for (void *p : allocations) (void)hipFree(p);
    allocations.clear();
+   if (windows) (void)hipFree(windows);
+   windows = nullptr;
 }
 
 using namespace ptx;
@@ -304,6 +337,7 @@ int ptx_scene_create(ptx_device *dev, const ptx_scene_desc *desc, ptx_scene **ou
    std::copy(img.root_box, img.root_box + 6, s->d.root_box);
    s->d.root_ref = img.root_ref;
    s->d.root_count = img.root_count;
+   s->root_axis = (img.root_ref & kLeafBit) ? 0u : img.nodes[img.root_ref].axis;
    s->d.num_tris = img.num_tris;
    s->d.max_stack = img.max_stack;
    s->d.has_sky = desc->has_skybox;
@@ -378,10 +412,12 @@ int ptx_intersect_closest_device_noalloc(ptx_scene *scene, const float *d_rays, 
       return set_error(PTX_ERR_HIP, "the no-allocation scratch of this device could not be located");
    const ptx_tuning t = tuning_of(dev);
    const uint32_t lds = lds_depth_of(t);
+   // window records only when an earlier query built them (building allocates)
+   const Node64 *win = built_windows(scene, t);
    // the spill takes (max_stack - lds) entries of 16 bytes per resident lane;
    // the grid shrinks to what the scratch holds, down to one block per CU
    const size_t per_block = scene->d.max_stack > lds ? size_t(scene->d.max_stack - lds) * 256 * sizeof(uint4) : 0;
-   uint32_t grid = closest_hit_grid(dev, t, lds, d_counters != nullptr || d_frames != nullptr);
+   uint32_t grid = closest_hit_grid(dev, t, lds, d_counters != nullptr || d_frames != nullptr, win != nullptr);
    if (per_block) {
       const size_t fit = (kNoAllocScratchBytes - 256) / per_block;
       if (fit < (size_t)dev->num_cus)
@@ -399,7 +435,7 @@ int ptx_intersect_closest_device_noalloc(ptx_scene *scene, const float *d_rays, 
    const uint32_t refill = t.refill_lanes ? std::min(64u, t.refill_lanes) : 1u;
    // the work counters are cleared by a kernel at the head of the launch (no memset)
    if (launch_closest_hit(scene->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill,
-                          leaf_lanes_of(t), static_cast<hipStream_t>(stream)) != 0) {
+                          leaf_lanes_of(t), win, scene->root_axis, static_cast<hipStream_t>(stream)) != 0) {
       (void)hipGetLastError();
       return set_error(PTX_ERR_HIP, "closest-hit launch failed");
    }
@@ -450,6 +486,7 @@ int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {
    if (t->shade_lanes > 64) return set_error(PTX_ERR_INVALID, "shade_lanes must be 0..64");
    if (t->leaf_lanes > 64) return set_error(PTX_ERR_INVALID, "leaf_lanes must be 0..64");
    if (t->plain_primary > PTX_PLAIN_RETRACE) return set_error(PTX_ERR_INVALID, "plain_primary must be 0, 1 or 2");
+   if (t->ray_windows > PTX_WINDOWS_OFF) return set_error(PTX_ERR_INVALID, "ray_windows must be 0, 1 or 2");
    set_tuning(dev, *t);
    return PTX_OK;
 }

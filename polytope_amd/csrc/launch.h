@@ -8,12 +8,18 @@
 
 namespace ptx {
 
+// windows: the scene's window records (launch_build_windows), or null to step
+// over sc.nodes; root_axis: the root's split axis (used with windows).
 int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits,
                        ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
                        uint32_t grid, uint32_t lds_depth, uint32_t refill_min, uint32_t leaf_min,
-                       hipStream_t stream);
-// Resident blocks per CU of the closest-hit kernel; extra = counters or frames requested.
-int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra);
+                       const Node64 *windows, uint32_t root_axis, hipStream_t stream);
+// Resident blocks per CU of the closest-hit kernel; extra = counters or frames
+// requested, windows = the form that steps over window records.
+int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra, bool windows);
+// Window records of the closest-hit kernel: win (4 * n_int Node64, 128-byte
+// aligned) from the scene's n_int interior records.
+int launch_build_windows(const Node64 *nodes, uint32_t n_int, Node64 *win, hipStream_t stream);
 
 // Size of the per-device static scratch of the no-allocation closest-hit entry
 // point: 256 bytes of work counters, then the deferred-stack spill.

@@ -157,8 +157,9 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
       img.root_box[k] = bvh.nodes[0].p0[k];
       img.root_box[3 + k] = bvh.nodes[0].p1[k];
    }
-   // 16 floats of padding: a traversal step loads 64 bytes from a leaf's first face
-   img.tris.assign(9 * np + 16, 0.f);
+   // 32 floats of padding: a traversal step loads 72 bytes (Node64 form) or 128
+   // bytes (window form) from a leaf's first face
+   img.tris.assign(9 * np + 32, 0.f);
    img.num_tris = (uint32_t)np;
    bool any_normals = false;
    for (const auto &m : meshes) any_normals |= m.has_normals;

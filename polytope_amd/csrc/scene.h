@@ -55,7 +55,14 @@ uint32_t lds_depth_of(const ptx_tuning &t);
 constexpr uint32_t kDefaultLeafLanes = 1;
 constexpr uint32_t kDefaultRenderLeafLanes = 16;
 uint32_t leaf_lanes_of(const ptx_tuning &t, bool render = false);
-uint32_t closest_hit_grid(const ptx_device *dev, const ptx_tuning &t, uint32_t lds, bool extra);
+uint32_t closest_hit_grid(const ptx_device *dev, const ptx_tuning &t, uint32_t lds, bool extra, bool windows);
+// The scene's window records for the closest-hit kernel, built on first use
+// when the tuning asks for them (ray_windows = ON); null when it does not, or
+// when they could not be built (the kernel then steps over the Node64 records,
+// with the same results).  Allocates on first use.
+const Node64 *scene_windows(ptx_scene *s, const ptx_tuning &t);
+// The window records if already built (never allocates), else null.
+const Node64 *built_windows(ptx_scene *s, const ptx_tuning &t);
 size_t closest_hit_scratch_bytes(const ptx_scene *s, uint32_t grid, uint32_t lds);
 int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
                     ptx_hit_frame *d_frames, hipStream_t stream, const ptx_tuning &t);
@@ -98,5 +105,9 @@ struct ptx_scene {
    uint32_t num_meshes = 0;
    bool has_glossy = false;          // a mesh that is not a light has PTX_BRDF_GLOSSY
    bool has_texture = false;         // a Lambert mesh that is not a light has a texture
+   uint32_t root_axis = 0;           // split axis of the root when it is interior
+   std::mutex windows_mu;            // guards windows and windows_failed
+   ptx::Node64 *windows = nullptr;   // closest-hit window records (scene_windows), owned
+   bool windows_failed = false;      // their allocation failed once: not retried
    ~ptx_scene();                     // frees every device buffer in `allocations`
 };

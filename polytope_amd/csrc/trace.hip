@@ -47,15 +47,20 @@ __device__ __forceinline__ void write_result(const Trav &s, const DevScene &sc, 
    }
 }
 
-template <bool kExtra>
+// kWin: step over the scene's window records (win, root_cnt = the root's
+// split axis when it is interior) instead of its Node64 records.
+template <bool kExtra, bool kWin>
 struct RayQuery {
    struct Ctx {};
    static constexpr bool kStaged = true;
+   static constexpr bool kWindows = kWin;
    DevScene sc;
    const float *rays;
    ptx_hit *hits;
    ptx_ray_counters *counters;
    ptx_hit_frame *frames;
+   const Node64 *win;
+   uint32_t root_cnt;
    __device__ __forceinline__ void load(uint32_t id, Fetched &f) const {
       const float *rp = rays + 6 * (size_t)id;
 #pragma unroll
@@ -70,8 +75,9 @@ struct RayQuery {
    __device__ __forceinline__ void begin_root(Ctx &, Trav &s) const {
       begin(s, sc.root_box, sc.root_ref, sc.root_count);
    }
-   __device__ __forceinline__ uint32_t root_count() const { return sc.root_count; }
+   __device__ __forceinline__ uint32_t root_count() const { return root_cnt; }
    __device__ __forceinline__ const Node64 *nodes(const Ctx &) const { return sc.nodes; }
+   __device__ __forceinline__ const Node64 *windows() const { return win; }
    __device__ __forceinline__ const float *tris(const Ctx &) const { return sc.tris; }
    __device__ __forceinline__ bool debug_pixel(const Ctx &) const { return false; }
    __device__ __forceinline__ void finish(uint32_t id, const Ctx &, const Trav &s) const {
@@ -79,13 +85,39 @@ struct RayQuery {
    }
 };
 
-template <int kLds, bool kExtra>
+template <int kLds, bool kExtra, bool kWin>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n, ptx_hit *__restrict__ hits,
               ptx_ray_counters *counters, ptx_hit_frame *frames, uint32_t *work, uint4 *spill, uint32_t refill_min,
-              uint32_t leaf_min) {
-   const RayQuery<kExtra> q{sc, rays, hits, counters, frames};
+              uint32_t leaf_min, const Node64 *win, uint32_t root_cnt) {
+   const RayQuery<kExtra, kWin> q{sc, rays, hits, counters, frames, win, root_cnt};
    persistent_trace<kLds>(q, n, work, spill, refill_min, leaf_min, sc.max_stack);
+}
+
+// Node64 record i with the count slot of each interior child replaced by that
+// child's split axis.
+__device__ __forceinline__ Node64 window_record(const Node64 *nodes, uint32_t i) {
+   Node64 r = nodes[i];
+   uint32_t c0 = r.counts & 0xFFFFu, c1 = r.counts >> 16;
+   if (!(r.ref[0] & kLeafBit)) c0 = nodes[r.ref[0]].axis;
+   if (!(r.ref[1] & kLeafBit)) c1 = nodes[r.ref[1]].axis;
+   r.counts = c0 | (c1 << 16);
+   return r;
+}
+
+// One thread per window: window w of interior node i is win[4 i + 2 w] (the
+// node) and win[4 i + 2 w + 1] (the child visited first for direction sign w
+// on the node's axis: child 1 when w = 1, child 0 when w = 0; zeros for a leaf).
+__global__ void k_build_windows(const Node64 *__restrict__ nodes, uint32_t n_int, Node64 *__restrict__ win) {
+   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= 2 * (size_t)n_int) return;
+   const uint32_t i = (uint32_t)(t >> 1), w = (uint32_t)(t & 1);
+   const Node64 a = window_record(nodes, i);
+   const uint32_t child = a.ref[w];
+   Node64 b{};
+   if (!(child & kLeafBit)) b = window_record(nodes, child);
+   win[2 * t] = a;
+   win[2 * t + 1] = b;
 }
 
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
@@ -129,25 +161,45 @@ int noalloc_scratch_address(int ordinal, void **ptr) {
 // for (max_stack - lds) * grid * 256 entries.
 namespace {
 
-template <int kLds>
-void launch_lds(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
-                ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill, uint32_t grid, uint32_t refill_min,
-                uint32_t leaf_min, hipStream_t stream) {
+template <int kLds, bool kWin>
+void launch_form(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
+                 ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill, uint32_t grid, uint32_t refill_min,
+                 uint32_t leaf_min, const Node64 *win, uint32_t root_cnt, hipStream_t stream) {
    if (d_counters || d_frames)
-      hipLaunchKernelGGL((dev::k_closest_hit<kLds, true>), dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                         d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+      hipLaunchKernelGGL((dev::k_closest_hit<kLds, true, kWin>), dim3(grid), dim3(256), 0, stream, sc, d_rays, n,
+                         d_hits, d_counters, d_frames, d_work, d_spill, refill_min, leaf_min, win, root_cnt);
    else
-      hipLaunchKernelGGL((dev::k_closest_hit<kLds, false>), dim3(grid), dim3(256), 0, stream, sc, d_rays, n, d_hits,
-                         d_counters, d_frames, d_work, d_spill, refill_min, leaf_min);
+      hipLaunchKernelGGL((dev::k_closest_hit<kLds, false, kWin>), dim3(grid), dim3(256), 0, stream, sc, d_rays, n,
+                         d_hits, d_counters, d_frames, d_work, d_spill, refill_min, leaf_min, win, root_cnt);
 }
 
 template <int kLds>
-int blocks_lds(bool extra) {
+void launch_lds(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits, ptx_ray_counters *d_counters,
+                ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill, uint32_t grid, uint32_t refill_min,
+                uint32_t leaf_min, const Node64 *win, uint32_t root_axis, hipStream_t stream) {
+   if (win) {
+      // the root's count slot: its face count when it is a leaf, else its split axis
+      const uint32_t root_cnt = (sc.root_ref & kLeafBit) ? sc.root_count : root_axis;
+      launch_form<kLds, true>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min,
+                              leaf_min, win, root_cnt, stream);
+   } else {
+      launch_form<kLds, false>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min,
+                               leaf_min, nullptr, sc.root_count, stream);
+   }
+}
+
+template <int kLds, bool kWin>
+int blocks_form(bool extra) {
    int b = 0;
    const hipError_t e =
-         extra ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_closest_hit<kLds, true>, 256, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_closest_hit<kLds, false>, 256, 0);
+         extra ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_closest_hit<kLds, true, kWin>, 256, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_closest_hit<kLds, false, kWin>, 256, 0);
    return e == hipSuccess && b > 0 ? b : 1;
+}
+
+template <int kLds>
+int blocks_lds(bool extra, bool windows) {
+   return windows ? blocks_form<kLds, true>(extra) : blocks_form<kLds, false>(extra);
 }
 
 }  // namespace
@@ -155,36 +207,45 @@ int blocks_lds(bool extra) {
 int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_hit *d_hits,
                        ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
                        uint32_t grid, uint32_t lds_depth, uint32_t refill_min, uint32_t leaf_min,
-                       hipStream_t stream) {
+                       const Node64 *windows, uint32_t root_axis, hipStream_t stream) {
    hipLaunchKernelGGL(dev::k_zero_u32, dim3(1), dim3(64), 0, stream, d_work, (uint32_t)dev::kSegments);
    if (n == 0) return 0;
    switch (lds_depth) {
       case 8:
-         launch_lds<8>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min, stream);
+         launch_lds<8>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min,
+                       windows, root_axis, stream);
          break;
       case 16:
          launch_lds<16>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min,
-                        stream);
+                        windows, root_axis, stream);
          break;
       case 24:
          launch_lds<24>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min,
-                        stream);
+                        windows, root_axis, stream);
          break;
       default:
          launch_lds<12>(sc, d_rays, n, d_hits, d_counters, d_frames, d_work, d_spill, grid, refill_min, leaf_min,
-                        stream);
+                        windows, root_axis, stream);
          break;
    }
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra) {
+int closest_hit_blocks_per_cu(uint32_t lds_depth, bool extra, bool windows) {
    switch (lds_depth) {
-      case 8: return blocks_lds<8>(extra);
-      case 16: return blocks_lds<16>(extra);
-      case 24: return blocks_lds<24>(extra);
-      default: return blocks_lds<12>(extra);
+      case 8: return blocks_lds<8>(extra, windows);
+      case 16: return blocks_lds<16>(extra, windows);
+      case 24: return blocks_lds<24>(extra, windows);
+      default: return blocks_lds<12>(extra, windows);
    }
+}
+
+int launch_build_windows(const Node64 *nodes, uint32_t n_int, Node64 *win, hipStream_t stream) {
+   if (n_int == 0) return 0;
+   const size_t threads = 2 * (size_t)n_int;
+   hipLaunchKernelGGL(dev::k_build_windows, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, stream, nodes,
+                      n_int, win);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // namespace ptx

@@ -122,7 +122,7 @@ __device__ __forceinline__ const PTX_GLOBAL T *as_global(const T *p) {
 // interior record, or the leaf's first face and the start of its second), plus
 // 8 bytes for a leaf's second face.  The operands land in the same registers
 // whatever the kind, so no lane waits on its loads before the compute phase.
-// A one-face leaf reads 28 bytes past its face: the face array carries 64
+// A one-face leaf reads 28 bytes past its face: the face array carries 128
 // bytes of padding.
 __device__ __forceinline__ void fetch(const Trav &s, const Node64 *nodes, const float *tris, Fetched &f) {
    const bool leaf = (s.cur & kLeafBit) != 0;
@@ -142,41 +142,51 @@ __device__ __forceinline__ void fetch(const Trav &s, const Node64 *nodes, const 
    }
 }
 
-// One step on fetched operands: test up to two faces of the leaf s.cur, or the
-// two children of the interior node s.cur.  Returns false when the traversal
-// has finished.
-template <int kLds>
-__device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<kLds> &st, bool debug_pixel) {
-   if (s.cur & kLeafBit) {
-      const uint32_t first = s.cur & ~kLeafBit;
-      const bool two = s.cnt > 1;
-      if (test_triangle(f.q, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
-         s.hit = first;
-         s.after = 0;
-      }
-      s.after++;
-      if (two) {
-         if (test_triangle(f.q + 9, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
-            s.hit = first + 1;
-            s.after = 0;
-         }
-         s.after++;
-      }
-      const uint32_t done = two ? 2u : 1u;
-      s.tris += done;
-      if (s.cnt > done) {            // more faces in this leaf: continue it next step
-         s.cur += done;
-         s.cnt -= done;
-         return true;
-      }
-      return pop_next(s, st);
+// Operands of a step over window records: a 128-byte window (the node's record
+// and its first child's) or 128 bytes from a leaf's first face (three faces
+// and part of a fourth).
+struct FetchedWin {
+   float q[32];
+};
+
+// Window records: four Node64 per interior node i, at win[4 i + 2 w]: the
+// node's record, then the record of the child the traversal visits first when
+// the ray's direction sign on the node's split axis is w (the high child when
+// w = 1, bvh.cpp:510-517), or zeros when that child is a leaf.  In these
+// records the count slot of an interior child holds the child's split axis, so
+// a lane that arrives at an interior node (from its parent, the stack or the
+// root) knows which window to load before it loads anything.  Each window is
+// one 128-byte aligned block, one L2 line.  Every lane issues the same eight
+// 16-byte loads from its own address whatever its node's kind, so no lane
+// waits on its loads before the compute phase; the face array's 128 bytes of
+// padding cover the 92 bytes a one-face leaf at its end reads past its face.
+__device__ __forceinline__ void fetch_window(const Trav &s, const Node64 *win, const float *tris, FetchedWin &f) {
+   const bool leaf = (s.cur & kLeafBit) != 0;
+   const PTX_GLOBAL float *p =
+         as_global(leaf ? tris + 9 * (size_t)(s.cur & ~kLeafBit)
+                        : reinterpret_cast<const float *>(win + 4 * (size_t)s.cur + 2 * ((s.r.nk >> s.cnt) & 1u)));
+#pragma unroll
+   for (int k = 0; k < 8; k++) {
+      const f32x4u v = *reinterpret_cast<const PTX_GLOBAL f32x4u *>(p + 4 * k);
+      f.q[4 * k] = v.x; f.q[4 * k + 1] = v.y; f.q[4 * k + 2] = v.z; f.q[4 * k + 3] = v.w;
    }
+}
+
+enum : int { kStepDone = 0, kStepFirst = 1, kStepOther = 2 };
+
+// The two children of the interior node s.cur from its record q[0..15]: the
+// child visited first is entered when its box is hit, the other is deferred
+// with its entry distance (pop-time test) or entered when only it is hit,
+// else the stack is popped.  Returns kStepFirst when the lane entered the
+// first child, kStepDone when the traversal has finished, else kStepOther.
+template <int kLds>
+__device__ __forceinline__ int interior_step(Trav &s, const float *q, const Stack<kLds> &st) {
    bool in0, in1;
    float n0, f0, n1, f1;
-   slab(f.q[0], f.q[1], f.q[2], f.q[3], f.q[4], f.q[5], s.r, in0, n0, f0);
-   slab(f.q[6], f.q[7], f.q[8], f.q[9], f.q[10], f.q[11], s.r, in1, n1, f1);
-   const uint32_t mref0 = __float_as_uint(f.q[12]), mref1 = __float_as_uint(f.q[13]);
-   const uint32_t mcounts = __float_as_uint(f.q[14]), maxis = __float_as_uint(f.q[15]);
+   slab(q[0], q[1], q[2], q[3], q[4], q[5], s.r, in0, n0, f0);
+   slab(q[6], q[7], q[8], q[9], q[10], q[11], s.r, in1, n1, f1);
+   const uint32_t mref0 = __float_as_uint(q[12]), mref1 = __float_as_uint(q[13]);
+   const uint32_t mcounts = __float_as_uint(q[14]), maxis = __float_as_uint(q[15]);
    const bool h0 = in0 || (n0 <= f0 && n0 <= s.min_t);
    const bool h1 = in1 || (n1 <= f1 && n1 <= s.min_t);
    // bvh.cpp:510-517: with neg_dir[axis] the high child (child 1) is popped first.
@@ -196,17 +206,65 @@ __device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<k
       }
       s.cur = fref;
       s.cnt = fcnt;
-      return true;
+      return kStepFirst;
    }
    s.bb_misses++;
    if (sh) {
       s.bb_hits++;
       s.cur = sref;
       s.cnt = scnt;
-      return true;
+      return kStepOther;
    }
    s.bb_misses++;
+   return pop_next(s, st) ? kStepOther : kStepDone;
+}
+
+// Up to kMax faces of the leaf s.cur (face k at q[9 k .. 9 k + 8]), in order.
+// Returns false when the traversal has finished.
+template <int kLds, int kMax>
+__device__ __forceinline__ bool leaf_step(Trav &s, const float *q, const Stack<kLds> &st, bool debug_pixel) {
+   const uint32_t first = s.cur & ~kLeafBit;
+   const uint32_t n = s.cnt < (uint32_t)kMax ? s.cnt : (uint32_t)kMax;
+#pragma unroll
+   for (int k = 0; k < kMax; k++) {
+      if (k == 0 || (uint32_t)k < n) {
+         if (test_triangle(q + 9 * k, s.r, s.min_t, s.hu, s.hv, s.hw, debug_pixel)) {
+            s.hit = first + k;
+            s.after = 0;
+         }
+         s.after++;
+      }
+   }
+   s.tris += n;
+   if (s.cnt > n) {               // more faces in this leaf: continue it next step
+      s.cur += n;
+      s.cnt -= n;
+      return true;
+   }
    return pop_next(s, st);
+}
+
+// One step over a window: the node's two children, then, when the lane
+// entered the first child and it is interior, that child's two children from
+// the window's second record; at a leaf, up to three faces.  The same tests in
+// the same order as the advance() steps they replace, so hits, counters and
+// stack contents are the same.
+template <int kLds>
+__device__ __forceinline__ bool advance_window(Trav &s, const FetchedWin &f, const Stack<kLds> &st,
+                                               bool debug_pixel) {
+   if (s.cur & kLeafBit) return leaf_step<kLds, 3>(s, f.q, st, debug_pixel);
+   int r = interior_step(s, f.q, st);
+   if (r == kStepFirst && !(s.cur & kLeafBit)) r = interior_step(s, f.q + 16, st);
+   return r != kStepDone;
+}
+
+// One step on fetched operands: test up to two faces of the leaf s.cur, or the
+// two children of the interior node s.cur.  Returns false when the traversal
+// has finished.
+template <int kLds>
+__device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<kLds> &st, bool debug_pixel) {
+   if (s.cur & kLeafBit) return leaf_step<kLds, 2>(s, f.q, st, debug_pixel);
+   return interior_step(s, f.q, st) != kStepDone;
 }
 
 constexpr int kSegments = 8;
@@ -264,6 +322,18 @@ struct staged_policy<P, decltype(void(P::kStaged))> {
    static constexpr bool value = P::kStaged;
 };
 
+// Policies that set kWindows step over window records (fetch_window,
+// advance_window) from windows(); they start at the root with s.cnt = the
+// root's split axis when the root is interior (root_count()).
+template <class P, class = void>
+struct window_policy {
+   static constexpr bool value = false;
+};
+template <class P>
+struct window_policy<P, decltype(void(P::kWindows))> {
+   static constexpr bool value = P::kWindows;
+};
+
 // Ray set-up (reciprocal direction, Wald shear, root box test: six correctly
 // rounded divisions and a slab test) done for a whole batch of ids at once:
 // lane j of the wave loads and prepares ray base + j, and lanes that need a
@@ -309,7 +379,9 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
    uint32_t id = 0;
    refill_min = min(refill_min, 64u);     // a wave that is all idle always refills
    constexpr bool kStaged = staged_policy<Policy>::value;
+   constexpr bool kWin = window_policy<Policy>::value;
    static_assert(!kStaged || std::is_empty<typename Policy::Ctx>::value, "staged policies carry no context");
+   static_assert(!kWin || kStaged, "window records are stepped by staged policies only");
    Staged g;
    g.ox = g.oy = g.oz = g.dx = g.dy = g.dz = g.ix = g.iy = g.iz = g.Sx = g.Sy = g.Sz = 0.f;
    g.nk = 0;
@@ -404,6 +476,15 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
       const bool hold = at_leaf && (uint32_t)__popcll(__ballot(at_leaf)) < leaf_min &&
                         __any(mode != kIdle && !at_leaf);
       const uint32_t m = hold ? kHeld : mode;
+      if constexpr (kWin) {
+         FetchedWin f;
+         if (m == kActive) fetch_window(s, P.windows(), P.tris(ctx), f);
+         if (m == kActive && !advance_window<kLds>(s, f, st, P.debug_pixel(ctx))) {
+            P.finish(id, ctx, s);
+            mode = kIdle;
+         }
+         continue;
+      }
       Fetched f;
       if (!kStaged && m == kPending) {
          P.load(id, f);

@@ -204,3 +204,62 @@ def test_gpu_full_size_closest_hit_sample(dev):
     idx = np.arange(0, len(rays), 61)
     ref = O.OracleScene(meshes).intersect(rays[idx])
     assert_same_hits(h[idx], c[idx], None, ref, frames=False)
+
+
+def _query_entry(entry, scene, rays):
+    """hits, counters and frames from one C-ABI closest-hit entry point."""
+    import ctypes as C  # noqa: F401  (the library's argtypes come from _abi)
+    from polytope_amd import PtxError, _abi
+    n = rays.shape[0]
+    lib = _abi.load()
+    r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).cuda()
+    hits = torch.full((max(n, 1) * 6,), -7, dtype=torch.int32, device="cuda")
+    cnt = torch.full((max(n, 1) * 3,), -7, dtype=torch.int32, device="cuda")
+    frm = torch.full((max(n, 1) * 15,), -7.0, dtype=torch.float32, device="cuda")
+    rc = getattr(lib, entry)(scene.handle, r.data_ptr(), n, hits.data_ptr(), cnt.data_ptr(), frm.data_ptr(),
+                             torch.cuda.current_stream().cuda_stream)
+    if rc != _abi.PTX_OK:
+        raise PtxError(rc, lib.ptx_last_error().decode())
+    torch.cuda.synchronize()
+    return (hits.cpu().numpy().view(HIT_DTYPE)[:n].copy(), cnt.cpu().numpy().view(COUNTER_DTYPE)[:n].copy(),
+            frm.cpu().numpy().view(FRAME_DTYPE)[:n].copy())
+
+
+def _window_cases():
+    yield "workload_subdiv6", S.closest_hit_workload(6, 1 << 17, seed=7)[:2]
+    v, f = S.icosphere(6, noise=0.3, seed=4)          # deep tree: the stack spills at LDS depth 8
+    m = [S.MeshData(v, f)]
+    lo, hi = S.scene_bounds(m)
+    yield "deep_noise", (m, S.rays_incoherent(lo, hi, 30000, 3))
+    for name, meshes, _sky, rays in _scene_cases():
+        if name in ("instances", "sliver160"):
+            yield name, (meshes, rays)
+
+
+@pytest.mark.parametrize("lds", [8, 12])
+@pytest.mark.parametrize("case", list(_window_cases()), ids=lambda c: c[0])
+def test_gpu_window_and_node_record_forms_agree(dev, case, lds):
+    # The window form (ray_windows=1: a node's record and its first-visited
+    # child's in one 128-byte block, up to three faces per leaf step) against
+    # the one-record form and the oracle: hits, counters and frames bit for
+    # bit.  The first query on a fresh scene through the no-allocation entry
+    # point runs the one-record form (it never builds the windows); the device
+    # entry point builds them, and the no-allocation entry point then uses them.
+    name, (meshes, rays) = case
+    print(f"case windows:{name} lds={lds} rays={len(rays)}", flush=True)
+    ref = O.OracleScene(meshes).intersect(rays)
+    dev.set_tuning(lds_stack=lds, ray_windows=1)
+    try:
+        sc = Scene(dev, meshes)
+        node_form = _query_entry("ptx_intersect_closest_device_noalloc", sc, rays)
+        win_form = _query_entry("ptx_intersect_closest_device", sc, rays)
+        win_noalloc = _query_entry("ptx_intersect_closest_device_noalloc", sc, rays)
+        dev.set_tuning(lds_stack=lds, ray_windows=2)
+        off_form = _query_entry("ptx_intersect_closest_device", sc, rays)
+        sc.close()
+    finally:
+        dev.set_tuning()
+    for a, b in ((node_form, win_form), (off_form, win_form), (win_noalloc, win_form)):
+        for x, y in zip(a, b):
+            assert x.tobytes() == y.tobytes()
+    assert_same_hits(*win_form, ref)
