@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--serving-loaded-requests", type=int, default=64,
                     help="second stream at a rate near the GPU's capacity")
     ap.add_argument("--serving-loaded-rate", type=float, default=64.0)
+    ap.add_argument("--render-batched-only", action="store_true",
+                    help="render section: the batched form and its accounting only (profiling runs)")
     ap.add_argument("--skip-accounting", action="store_true",
                     help="skip the render workload's ray and byte accounting (oracle on a row sample)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -193,6 +195,18 @@ def render_section(dev, args):
     films_b = render_batch(batch)
     t_batched = time.perf_counter() - tb
     print(f"[bench] batched: {samples / t_batched / 1e6:.1f} Msamples/s", file=sys.stderr, flush=True)
+    if args.render_batched_only:
+        accounting = None
+        if not args.skip_accounting:
+            import oracle as O
+            orc = O.OracleScene(meshes, sky, bvh=scene.bvh())
+            accounting = render_accounting(orc, jobs, t_batched, host_threads(args))
+            del orc
+        scene.close()
+        return {"workload": f"icosphere subdiv {args.render_subdiv} x {args.render_instances + 1} instances + ground "
+                            f"({sum(m.num_faces for m in meshes)} tris), 16 jobs, {args.render_spp} spp, depth 8",
+                "samples": samples, "batched_s": round(t_batched, 3),
+                "batched_samples_per_s": round(samples / t_batched, 1), "accounting": accounting}
     dev.set_tuning(render_mode=_abi.RENDER_WAVEFRONT, paths_per_wave=args.render_paths, leaf_lanes=args.leaf_lanes)
     render_batch([(scene, jobs[0][0], 1, 8, 1)])
     tw = time.perf_counter()
@@ -508,7 +522,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.skip_render:
         out["render"] = render_section(dev, args)
-        out["render_cpu_runnable"] = plain_cpu_render_section(dev, args, host_threads(args))
+        if not args.render_batched_only:
+            out["render_cpu_runnable"] = plain_cpu_render_section(dev, args, host_threads(args))
     if not args.skip_serving:
         out["serving"] = serving_section(dev, args, args.serving_requests, args.serving_rate)
         out["serving_loaded"] = serving_section(dev, args, args.serving_loaded_requests, args.serving_loaded_rate)

@@ -233,14 +233,16 @@ int ptx_intersect_closest_device_noalloc(ptx_scene *scene, const float *d_rays, 
                                          ptx_hit *d_hits, ptx_ray_counters *d_counters,
                                          ptx_hit_frame *d_frames, void *stream);
 
-/* Plain GPU render of one job (one launch series per bounce, as the reference
- * CUDA path), film_rgb = W*H*3 floats on the host (BoxFilter::Output values). */
+/* Plain GPU render of one job: one launch per sample, each pixel's path in one
+ * thread with every bounce inline (the launch shape of the reference's CUDA
+ * path); each pixel's primary ray is traced once unless ptx_tuning.plain_primary
+ * = RETRACE.  film_rgb = W*H*3 floats on the host (BoxFilter::Output values). */
 int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb);
 /* Batched render: all jobs' samples are merged into rounds of path slots that
- * share the GPU.  By default a round is one persistent path-regeneration launch
- * (every lane takes the next path slot as soon as its path ends); the wavefront
- * form keeps the paths in queues and replays one captured hipGraph per bounce.
- * films[i] = W_i*H_i*3 host floats. */
+ * share the GPU.  A round is one persistent path-regeneration launch (every
+ * lane takes the next path slot as soon as its path ends).  The wavefront form
+ * (render_mode = WAVEFRONT, a comparison form) keeps the paths in queues and
+ * replays one captured hipGraph per bounce.  films[i] = W_i*H_i*3 host floats. */
 int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_t n_jobs,
                      float *const *films);
 
@@ -269,8 +271,10 @@ typedef struct ptx_tuning {
    uint32_t lds_stack;       /* LDS stack entries per lane: 8, 12, 16 or 24           */
    uint32_t refill_lanes;    /* idle lanes that trigger a wave's ray refill (1..64)   */
    uint32_t blocks_per_cu;   /* persistent blocks per CU; 0 = occupancy limit         */
-   uint32_t render_mode;     /* PTX_RENDER_*: AUTO = regeneration where a round fits  */
-                             /* its LDS tables (64 chunks, 64 scenes), else wavefront */
+   uint32_t render_mode;     /* PTX_RENDER_*: AUTO and REGEN = path regeneration     */
+                             /* (rounds of at most 64 chunks; a batch of more than  */
+                             /* 64 scenes runs as consecutive groups of at most 64); */
+                             /* WAVEFRONT = path queues, kept as a comparison form   */
    uint32_t shade_lanes;     /* regeneration: lanes of a wave that must wait for a    */
                              /* shading step before the wave shades (1..64; 0 = 16)  */
    uint32_t leaf_lanes;      /* traversal: lanes of a wave that must reach a leaf     */

@@ -494,7 +494,31 @@ int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_
    std::lock_guard<std::mutex> lock(dev->mu);
    RenderEngine *eng = engine_of(dev);
    eng->tun = tuning_of(dev);
-   return eng->run_batch(rj, uniq, out);
+   const uint32_t max_scenes = paths_max_scenes();
+   if (uniq.size() <= max_scenes || eng->tun.render_mode == PTX_RENDER_WAVEFRONT) return eng->run_batch(rj, uniq, out);
+   // More scenes than the regeneration kernel's scene table holds: the jobs run
+   // in job order in groups of at most that many scenes, each group one batch
+   // (a job's film does not depend on the jobs rendered beside it).
+   for (uint32_t i = 0; i < n_jobs;) {
+      std::vector<const ptx_scene *> group;
+      std::map<const ptx_scene *, uint32_t> group_index;
+      std::vector<RenderJob> group_jobs;
+      std::vector<float *> group_out;
+      for (; i < n_jobs; i++) {
+         const ptx_scene *sc = uniq[rj[i].scene_index];
+         auto it = group_index.find(sc);
+         if (it == group_index.end()) {
+            if (group.size() == max_scenes) break;
+            it = group_index.emplace(sc, (uint32_t)group.size()).first;
+            group.push_back(sc);
+         }
+         group_jobs.push_back(rj[i]);
+         group_jobs.back().scene_index = it->second;
+         group_out.push_back(out[i]);
+      }
+      if (int rc = eng->run_batch(group_jobs, group, group_out)) return rc;
+   }
+   return PTX_OK;
 }
 
 }  // extern "C"

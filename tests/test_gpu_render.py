@@ -230,3 +230,22 @@ def test_scene_and_server_outlive_their_device_handle():
     x = torch.ones(4, device="cuda") * 2
     torch.cuda.synchronize()
     assert float(x.sum()) == 8.0
+
+
+def test_render_batch_more_scenes_than_lds_table(dev):
+    # 70 scenes: more than the regeneration kernel's 64-entry scene table, so the
+    # batch runs as consecutive groups of at most 64 scenes in the default form;
+    # every film equals the job rendered alone and the wavefront form's
+    meshes, cam, sky = _lambert_scene(16, 12)
+    scenes = [Scene(dev, meshes, (sky[0] + i, sky[1], sky[2])) for i in range(70)]
+    jobs = [(scenes[i], cam, 1 + i % 2, 1 + i % 3, 300 + i) for i in range(70)]
+    print("case 70 scenes", flush=True)
+    grouped = render_batch(jobs)
+    queued = batch(dev, jobs, _abi.RENDER_WAVEFRONT)
+    for i in range(70):
+        assert np.array_equal(grouped[i].view(np.uint32), queued[i].view(np.uint32)), i
+    for i in (0, 63, 64, 69):
+        (alone,) = render_batch([jobs[i]])
+        assert np.array_equal(alone.view(np.uint32), grouped[i].view(np.uint32)), i
+    for s in scenes:
+        s.close()
