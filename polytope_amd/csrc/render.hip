@@ -1177,52 +1177,163 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             const uint32_t next = p_root<kNee>(S, s, res, slot, src, direct, mode);
             mode = next == kPIdle ? kPIdle : lw_with_state(mode, next);
          }
-      } else if (m == kPPrim) {
-         const LdsChunk &c = sch[lw_chunk(mode)];
-         const uint32_t local = slot - c.base;
-         const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
-         const uint32_t w0 = __float_as_uint(f.q[0]);
-         if (kNee) {
-            direct[0] = direct[1] = direct[2] = 0.f;
-            if (c.flags & PTX_JOB_NEE) mode |= kPNee;
-         }
-         if (w0 & 1u) {
-            write_res(res, slot, fin<kNee>(Radiance{f.q[1], f.q[2], f.q[3]}, direct, mode));
-            mode = kPIdle;
-         } else {
-            const DevScene &S = ssc[c.scene];
-            MeshRec M;
-            if (kTex && (w0 & 2u)) {               // textured Lambert: the texel rides in the record
-               M.brdf = PTX_BRDF_LAMBERT;
-               texel_refl(w0 >> 8, M.refl);
-            } else {
-               const PTX_GLOBAL uint32_t *mw = as_global(reinterpret_cast<const uint32_t *>(S.meshes + (w0 >> 2)));
-               M.brdf = (int32_t)mw[0];
-               M.refl[0] = __uint_as_float(mw[1]);
-               M.refl[1] = __uint_as_float(mw[2]);
-               M.refl[2] = __uint_as_float(mw[3]);
-               if (kGlossy) M.emission[0] = __uint_as_float(mw[5]);       // glossy roughness
+      } else {
+         // A path's start (from the pixel's primary record) and a hit's
+         // shading (Shade2 / Shade3) end in the same bounce: the branches
+         // below set up the vertex, and one copy of the bounce, the
+         // next-event set-up and the ray start then serves every lane that
+         // shades in this step, whichever branch it came through.  The
+         // vertex travels in registers the lane holds anyway: the frame and
+         // the incoming direction in f.q[1..15] (the primary record's
+         // layout), the material in t0 (BRDF), t1..t3 (reflectance) and s.cnt
+         // (glossy roughness).
+         bool shade = false;
+         uint32_t pixel = 0, sample = 0, b = 0, seed = 0;
+         uint32_t nb = 0;                         // the lane word after the bounce, state bits clear
+         uint32_t scene = 0;                      // index into the round's scene table
+         if (m == kPPrim) {
+            const LdsChunk &c = sch[lw_chunk(mode)];
+            const uint32_t local = slot - c.base;
+            sample = c.s0 + local / c.npix;
+            pixel = local % c.npix;
+            const uint32_t w0 = __float_as_uint(f.q[0]);
+            if (kNee) {
+               direct[0] = direct[1] = direct[2] = 0.f;
+               if (c.flags & PTX_JOB_NEE) mode |= kPNee;
             }
+            if (w0 & 1u) {
+               write_res(res, slot, fin<kNee>(Radiance{f.q[1], f.q[2], f.q[3]}, direct, mode));
+               mode = kPIdle;
+            } else {
+               if (kTex && (w0 & 2u)) {            // textured Lambert: the texel rides in the record
+                  float refl[3];
+                  texel_refl(w0 >> 8, refl);
+                  t0 = (uint32_t)PTX_BRDF_LAMBERT;
+                  t1 = refl[0]; t2 = refl[1]; t3 = refl[2];
+               } else {
+                  const PTX_GLOBAL uint32_t *mw =
+                        as_global(reinterpret_cast<const uint32_t *>(ssc[c.scene].meshes + (w0 >> 2)));
+                  t0 = mw[0];
+                  t1 = __uint_as_float(mw[1]);
+                  t2 = __uint_as_float(mw[2]);
+                  t3 = __uint_as_float(mw[3]);
+                  if (kGlossy) s.cnt = mw[5];      // glossy roughness
+               }
+               src[0] = src[1] = src[2] = 1.f;
+               const uint32_t x = pixel % c.width, y = pixel / c.width;
+               const uint32_t dbg = (x == 128 && y == 128) ? kPDbg : 0u;
+               nb = (mode & ~kPModeMask) | dbg | (1u << kPBounceShift);
+               b = 0u;
+               seed = c.seed;
+               scene = c.scene;
+               shade = true;
+            }
+         } else if (m == kPShade1) {
+            t0 = __float_as_uint(f.q[0]);
+            if (kTex) {
+               const uint32_t wh = __float_as_uint(f.q[7]);
+               s.r.oy = f.q[7];                    // the texture record's flags
+               if (wh & kTexTextured) {            // the texel's address, for Shade2's load
+                  const DevScene &S = ssc[sch[lw_chunk(mode)].scene];
+                  const uint32_t k = __float_as_uint(f.q[8]) + texel_index(&f.q[1], s.hu, s.hv, s.hw, wh);
+                  const uint64_t a = reinterpret_cast<uint64_t>(texel_pool(S) + k);
+                  s.r.ix = __uint_as_float((uint32_t)a);
+                  s.r.iy = __uint_as_float((uint32_t)(a >> 32));
+               }
+            }
+            mode = lw_with_state(mode, kPShade2);
+         } else if (m == kPShade2 || m == kPShade3) {
+            const LdsChunk &c = sch[lw_chunk(mode)];
+            const uint32_t bi = lw_bounce(mode);
+            bool go = true;
+            if (kTex && m == kPShade2 && (__float_as_uint(s.r.oy) & kTexTextured)) {
+               // textured Lambert (never an emitter): the texel is the reflectance
+               if (bi >= c.max_depth) {
+                  write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
+                  mode = kPIdle;
+                  go = false;
+               } else {
+                  float refl[3];
+                  texel_refl(__float_as_uint(f.q[0]), refl);
+                  t0 = (uint32_t)PTX_BRDF_LAMBERT;
+                  t1 = refl[0]; t2 = refl[1]; t3 = refl[2];
+                  if ((__float_as_uint(s.r.oy) & kTexNormals) && ssc[c.scene].normals) {
+                     mode = lw_with_state(mode, kPShade3);
+                     go = false;
+                  } else {
+#pragma unroll
+                     for (int k = 0; k < 9; k++) f.q[k] = f.q[9 + k];
+                  }
+               }
+            } else if (m == kPShade2) {
+               const int32_t brdf = __float_as_int(f.q[0]);
+               const int32_t is_light = __float_as_int(f.q[4]);
+               const uint32_t mflags = __float_as_uint(f.q[8]);
+               if (kGlossy) s.cnt = __float_as_uint(f.q[5]);          // glossy roughness, kept until the bounce
+               if (is_light) {
+                  write_res(res, slot, fin<kNee>(Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
+                                                          (f.q[7] + 0.f) * src[2]}, direct, mode));
+                  mode = kPIdle;
+                  go = false;
+               } else if (bi >= c.max_depth) {
+                  write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
+                  mode = kPIdle;
+                  go = false;
+               } else if ((mflags & kRecNormals) && ssc[c.scene].normals) {
+                  t0 = (uint32_t)brdf;
+                  t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
+                  mode = lw_with_state(mode, kPShade3);
+                  go = false;
+               } else {
+                  t0 = (uint32_t)brdf;
+                  t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
+#pragma unroll
+                  for (int k = 0; k < 9; k++) f.q[k] = f.q[9 + k];
+               }
+            }
+            if (go) {
+               Frame fr;
+               hit_frame_sel(&f.q[0], &f.q[9], m == kPShade3, s.r, s.hu, s.hv, s.hw, s.after, fr);
+               const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
+               float lin[3];
+               world_to_local(fr, d, lin);
+               f.q[1] = fr.lx; f.q[2] = fr.ly; f.q[3] = fr.lz;
+               f.q[4] = fr.t1x; f.q[5] = fr.t1y; f.q[6] = fr.t1z;
+               f.q[7] = fr.nx; f.q[8] = fr.ny; f.q[9] = fr.nz;
+               f.q[10] = fr.t2x; f.q[11] = fr.t2y; f.q[12] = fr.t2z;
+               f.q[13] = lin[0]; f.q[14] = lin[1]; f.q[15] = lin[2];
+               const uint32_t local = slot - c.base;
+               sample = c.s0 + local / c.npix;
+               pixel = local % c.npix;
+               b = bi;
+               seed = c.seed;
+               nb = (mode & ~kPModeMask) + (1u << kPBounceShift);
+               scene = c.scene;
+               shade = true;
+            }
+         }
+         if (shade) {
+            const DevScene &S = ssc[scene];
+            MeshRec M;
+            M.brdf = (int32_t)t0;
+            M.refl[0] = t1; M.refl[1] = t2; M.refl[2] = t3;
+            if (kGlossy) M.emission[0] = __uint_as_float(s.cnt);
             Frame fr;
             fr.lx = f.q[1]; fr.ly = f.q[2]; fr.lz = f.q[3];
             fr.t1x = f.q[4]; fr.t1y = f.q[5]; fr.t1z = f.q[6];
             fr.nx = f.q[7]; fr.ny = f.q[8]; fr.nz = f.q[9];
             fr.t2x = f.q[10]; fr.t2y = f.q[11]; fr.t2z = f.q[12];
             const float lin[3] = {f.q[13], f.q[14], f.q[15]};
-            src[0] = src[1] = src[2] = 1.f;
-            const uint32_t x = pixel % c.width, y = pixel / c.width;
-            const uint32_t dbg = (x == 128 && y == 128) ? kPDbg : 0u;
             const bool nee = kNee && (mode & kPNee) && M.brdf == PTX_BRDF_LAMBERT && S.num_lights > 0;
-            const float kd[3] = {M.refl[0], M.refl[1], M.refl[2]};
             const float s_before[3] = {src[0], src[1], src[2]};
             float wout[3];
-            bounce<kGlossy>(M, fr, lin, pixel, sample, 0u, c.seed, wout, src);
+            bounce<kGlossy>(M, fr, lin, pixel, sample, b, seed, wout, src);
             nodes = S.nodes;
             tris = S.tris;
             if (nee) {
                const float keep[3] = {src[0], src[1], src[2]};
                src[0] = s_before[0]; src[1] = s_before[1]; src[2] = s_before[2];
-               nee_vertex(fr, kd, wout, (mode & ~kPModeMask) | dbg | (1u << kPBounceShift));
+               nee_vertex(fr, M.refl, wout, nb);
                src[0] = keep[0]; src[1] = keep[1]; src[2] = keep[2];
                t2 = src_is_zero(src) ? 1.f : 0.f;
             } else if (src_is_zero(src)) {
@@ -1231,101 +1342,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
             } else {
                make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
                const uint32_t next = p_root<kNee>(S, s, res, slot, src, direct, mode);
-               mode = next == kPIdle ? kPIdle : (mode & ~kPModeMask) | dbg | (1u << kPBounceShift) | next;
-            }
-         }
-      } else if (m == kPShade1) {
-         t0 = __float_as_uint(f.q[0]);
-         if (kTex) {
-            const uint32_t wh = __float_as_uint(f.q[7]);
-            s.r.oy = f.q[7];                       // the texture record's flags
-            if (wh & kTexTextured) {               // the texel's address, for Shade2's load
-               const DevScene &S = ssc[sch[lw_chunk(mode)].scene];
-               const uint32_t k = __float_as_uint(f.q[8]) + texel_index(&f.q[1], s.hu, s.hv, s.hw, wh);
-               const uint64_t a = reinterpret_cast<uint64_t>(texel_pool(S) + k);
-               s.r.ix = __uint_as_float((uint32_t)a);
-               s.r.iy = __uint_as_float((uint32_t)(a >> 32));
-            }
-         }
-         mode = lw_with_state(mode, kPShade2);
-      } else if (m == kPShade2 || m == kPShade3) {
-         const LdsChunk &c = sch[lw_chunk(mode)];
-         const uint32_t b = lw_bounce(mode);
-         bool go = true;
-         if (kTex && m == kPShade2 && (__float_as_uint(s.r.oy) & kTexTextured)) {
-            // textured Lambert (never an emitter): the texel is the reflectance
-            if (b >= c.max_depth) {
-               write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
-               mode = kPIdle;
-               go = false;
-            } else {
-               float refl[3];
-               texel_refl(__float_as_uint(f.q[0]), refl);
-               t0 = (uint32_t)PTX_BRDF_LAMBERT;
-               t1 = refl[0]; t2 = refl[1]; t3 = refl[2];
-               if ((__float_as_uint(s.r.oy) & kTexNormals) && ssc[c.scene].normals) {
-                  mode = lw_with_state(mode, kPShade3);
-                  go = false;
-               } else {
-#pragma unroll
-                  for (int k = 0; k < 9; k++) f.q[k] = f.q[9 + k];
-               }
-            }
-         } else if (m == kPShade2) {
-            const int32_t brdf = __float_as_int(f.q[0]);
-            const int32_t is_light = __float_as_int(f.q[4]);
-            const uint32_t mflags = __float_as_uint(f.q[8]);
-            if (kGlossy) s.cnt = __float_as_uint(f.q[5]);             // glossy roughness, kept until the bounce
-            if (is_light) {
-               write_res(res, slot, fin<kNee>(Radiance{(f.q[5] + 0.f) * src[0], (f.q[6] + 0.f) * src[1],
-                                                       (f.q[7] + 0.f) * src[2]}, direct, mode));
-               mode = kPIdle;
-               go = false;
-            } else if (b >= c.max_depth) {
-               write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
-               mode = kPIdle;
-               go = false;
-            } else if ((mflags & kRecNormals) && ssc[c.scene].normals) {
-               t0 = (uint32_t)brdf;
-               t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
-               mode = lw_with_state(mode, kPShade3);
-               go = false;
-            } else {
-               t0 = (uint32_t)brdf;
-               t1 = f.q[1]; t2 = f.q[2]; t3 = f.q[3];
-#pragma unroll
-               for (int k = 0; k < 9; k++) f.q[k] = f.q[9 + k];
-            }
-         }
-         if (go) {
-            MeshRec M;
-            M.brdf = (int32_t)t0;
-            M.refl[0] = t1; M.refl[1] = t2; M.refl[2] = t3;
-            if (kGlossy) M.emission[0] = __uint_as_float(s.cnt);
-            Frame fr;
-            hit_frame_sel(&f.q[0], &f.q[9], m == kPShade3, s.r, s.hu, s.hv, s.hw, s.after, fr);
-            const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
-            const uint32_t local = slot - c.base;
-            const uint32_t sample = c.s0 + local / c.npix, pixel = local % c.npix;
-            const DevScene &S = ssc[c.scene];
-            const bool nee = kNee && (mode & kPNee) && M.brdf == PTX_BRDF_LAMBERT && S.num_lights > 0;
-            const float s_before[3] = {src[0], src[1], src[2]};
-            float lin[3], wout[3];
-            world_to_local(fr, d, lin);
-            bounce<kGlossy>(M, fr, lin, pixel, sample, b, c.seed, wout, src);
-            if (nee) {
-               const float keep[3] = {src[0], src[1], src[2]};
-               src[0] = s_before[0]; src[1] = s_before[1]; src[2] = s_before[2];
-               nee_vertex(fr, M.refl, wout, mode + (1u << kPBounceShift));
-               src[0] = keep[0]; src[1] = keep[1]; src[2] = keep[2];
-               t2 = src_is_zero(src) ? 1.f : 0.f;
-            } else if (src_is_zero(src)) {
-               write_res(res, slot, fin<kNee>(Radiance{0.f, 0.f, 0.f}, direct, mode));
-               mode = kPIdle;
-            } else {
-               make_ray(s.r, fr.lx, fr.ly, fr.lz, wout[0], wout[1], wout[2]);
-               const uint32_t next = p_root<kNee>(S, s, res, slot, src, direct, mode);
-               mode = next == kPIdle ? kPIdle : lw_with_state(mode + (1u << kPBounceShift), next);
+               mode = next == kPIdle ? kPIdle : nb | next;
             }
          }
       }
