@@ -24,7 +24,8 @@ namespace ptx {
 //   f[0..5]   child 0 box p0.xyz, p1.xyz
 //   f[6..11]  child 1 box p0.xyz, p1.xyz
 //   u[12]     child 0 ref, u[13] child 1 ref
-//   u[14]     child 0 count | child 1 count << 16   (0 for an interior child)
+//   u[14]     child 0 slot | child 1 slot << 16: a leaf child's face count,
+//             an interior child's split axis
 //   u[15]     split axis
 struct alignas(64) Node64 {
    float box[12];

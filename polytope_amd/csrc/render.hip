@@ -1004,8 +1004,8 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
    const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
                         gtid, gridDim.x * blockDim.x, max_stack};
-   RaySource srcq{nslots, blockIdx.x % kSegments, 0u, 0u, 0u, 0u, nslots == 0};
-   if (!srcq.done) srcq.request(work, lane);
+   RaySource srcq;
+   srcq.init(nslots, work, lane);
    Trav s;
    s.hit = kNone;
    s.after = 0;

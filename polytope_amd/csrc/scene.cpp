@@ -133,7 +133,11 @@ int layout_scene(const std::vector<WorldMesh> &meshes, const HostBvh &bvh, HostI
       }
       r.ref[0] = ref[lo];
       r.ref[1] = ref[hi];
-      r.counts = (uint32_t)c0.num_faces | ((uint32_t)c1.num_faces << 16);
+      // a leaf child's slot holds its face count, an interior child's its split
+      // axis (which the closest-hit kernel's window records are selected by)
+      const uint32_t k0 = c0.num_faces ? (uint32_t)c0.num_faces : (uint32_t)c0.flags;
+      const uint32_t k1 = c1.num_faces ? (uint32_t)c1.num_faces : (uint32_t)c1.flags;
+      r.counts = k0 | (k1 << 16);
       r.axis = b.flags;
    }
    // deepest chain of interior nodes bounds the deferred stack

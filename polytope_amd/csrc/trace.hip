@@ -94,28 +94,18 @@ k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n, ptx_hit *
    persistent_trace<kLds>(q, n, work, spill, refill_min, leaf_min, sc.max_stack);
 }
 
-// Node64 record i with the count slot of each interior child replaced by that
-// child's split axis.
-__device__ __forceinline__ Node64 window_record(const Node64 *nodes, uint32_t i) {
-   Node64 r = nodes[i];
-   uint32_t c0 = r.counts & 0xFFFFu, c1 = r.counts >> 16;
-   if (!(r.ref[0] & kLeafBit)) c0 = nodes[r.ref[0]].axis;
-   if (!(r.ref[1] & kLeafBit)) c1 = nodes[r.ref[1]].axis;
-   r.counts = c0 | (c1 << 16);
-   return r;
-}
-
 // One thread per window: window w of interior node i is win[4 i + 2 w] (the
-// node) and win[4 i + 2 w + 1] (the child visited first for direction sign w
-// on the node's axis: child 1 when w = 1, child 0 when w = 0; zeros for a leaf).
+// node's record) and win[4 i + 2 w + 1] (the record of the child visited first
+// for direction sign w on the node's axis: child 1 when w = 1, child 0 when
+// w = 0; zeros when that child is a leaf).
 __global__ void k_build_windows(const Node64 *__restrict__ nodes, uint32_t n_int, Node64 *__restrict__ win) {
    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= 2 * (size_t)n_int) return;
    const uint32_t i = (uint32_t)(t >> 1), w = (uint32_t)(t & 1);
-   const Node64 a = window_record(nodes, i);
+   const Node64 a = nodes[i];
    const uint32_t child = a.ref[w];
    Node64 b{};
-   if (!(child & kLeafBit)) b = window_record(nodes, child);
+   if (!(child & kLeafBit)) b = nodes[child];
    win[2 * t] = a;
    win[2 * t + 1] = b;
 }
@@ -208,7 +198,7 @@ int launch_closest_hit(const DevScene &sc, const float *d_rays, uint32_t n, ptx_
                        ptx_ray_counters *d_counters, ptx_hit_frame *d_frames, uint32_t *d_work, uint4 *d_spill,
                        uint32_t grid, uint32_t lds_depth, uint32_t refill_min, uint32_t leaf_min,
                        const Node64 *windows, uint32_t root_axis, hipStream_t stream) {
-   hipLaunchKernelGGL(dev::k_zero_u32, dim3(1), dim3(64), 0, stream, d_work, (uint32_t)dev::kSegments);
+   hipLaunchKernelGGL(dev::k_zero_u32, dim3(1), dim3(64), 0, stream, d_work, (uint32_t)dev::kWorkWords);
    if (n == 0) return 0;
    switch (lds_depth) {
       case 8:

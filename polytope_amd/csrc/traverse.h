@@ -152,10 +152,11 @@ struct FetchedWin {
 // Window records: four Node64 per interior node i, at win[4 i + 2 w]: the
 // node's record, then the record of the child the traversal visits first when
 // the ray's direction sign on the node's split axis is w (the high child when
-// w = 1, bvh.cpp:510-517), or zeros when that child is a leaf.  In these
-// records the count slot of an interior child holds the child's split axis, so
-// a lane that arrives at an interior node (from its parent, the stack or the
-// root) knows which window to load before it loads anything.  Each window is
+// w = 1, bvh.cpp:510-517), or zeros when that child is a leaf.  The count slot
+// of an interior child holds the child's split axis (in the Node64 records
+// too), so a lane that arrives at an interior node (from its parent, the stack
+// or the root) knows which window to load before it loads anything, and a
+// traversal can move between the two record forms at any step.  Each window is
 // one 128-byte aligned block, one L2 line.  Every lane issues the same eight
 // 16-byte loads from its own address whatever its node's kind, so no lane
 // waits on its loads before the compute phase; the face array's 128 bytes of
@@ -267,37 +268,59 @@ __device__ __forceinline__ bool advance(Trav &s, const Fetched &f, const Stack<k
    return interior_step(s, f.q, st) != kStepDone;
 }
 
-constexpr int kSegments = 8;
-constexpr uint32_t kBatch = 64;     // ray ids taken per atomic
+constexpr uint32_t kBatch = 64;        // ids handed to a wave at a time
+constexpr uint32_t kWorkWords = 8;     // u32 words of work counters a launch clears
 
-// Per-wave source of ray ids: eight segments of the ray list, one per XCD
-// (blockIdx % 8 is the XCD a block runs on), each drained by an atomic counter
-// in batches of 64.  Lane 0 requests the next batch while the current one is
-// handed out, so the atomic's latency overlaps traversal steps.
+// Per-wave source of work ids, in batches of 64 consecutive ids.  The first
+// three quarters of the batches are dealt out without atomics: wave w of W
+// takes batches w, w + W, w + 2W, ..., so every wave samples the whole id
+// range and a coherent batch (neighbouring pixels) still lands on one wave.
+// The last quarter is handed out by one atomic counter (work[0]), which evens
+// out what the static share leaves uneven; lane 0 requests the next dynamic
+// batch while the current one is handed out (and the first one while the
+// wave takes its last static batch), so the atomic's latency overlaps
+// traversal steps.  When the counter passes the end, the wave is done: no
+// wave ever waits on more than one atomic at the end of a launch.
 struct RaySource {
-   uint32_t n, seg, tries, pool_base, pool_left, prefetch;   // wave-uniform except prefetch (lane 0)
+   uint32_t n;                    // ids
+   uint32_t batches, static_batches;
+   uint32_t waves;                // waves in the grid
+   uint32_t next;                 // this wave's next static batch
+   uint32_t pool_base, pool_left;
+   uint32_t prefetch;             // pending dynamic request's result (lane 0)
    bool done;
-   __device__ __forceinline__ uint32_t seg_lo(uint32_t sg) const { return (uint32_t)((uint64_t)n * sg / kSegments); }
-   __device__ __forceinline__ uint32_t seg_size(uint32_t sg) const { return seg_lo(sg + 1) - seg_lo(sg); }
-   __device__ __forceinline__ void request(uint32_t *work, uint32_t lane) {
-      if (lane == 0) prefetch = atomicAdd(&work[seg], kBatch);
+   __device__ __forceinline__ void init(uint32_t n_ids, uint32_t *work, uint32_t lane) {
+      n = n_ids;
+      batches = (n + kBatch - 1) / kBatch;
+      static_batches = batches - batches / 4;
+      waves = gridDim.x * (blockDim.x / 64u);
+      next = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+      pool_base = pool_left = 0;
+      prefetch = 0;
+      done = n == 0;
+      if (!done && next >= static_batches) request(work, lane);
    }
-   // Makes pool_left > 0 unless every segment is drained (then done = true).
+   __device__ __forceinline__ void request(uint32_t *work, uint32_t lane) {
+      if (lane == 0) prefetch = atomicAdd(&work[0], 1u);
+   }
+   // Makes pool_left > 0 unless every batch is taken (then done = true).
    __device__ __forceinline__ void refill(uint32_t *work, uint32_t lane) {
-      while (pool_left == 0 && !done) {
-         const uint32_t b = __builtin_amdgcn_readlane(prefetch, 0);
-         const uint32_t size = seg_size(seg);
-         if (b < size) {
-            pool_base = seg_lo(seg) + b;
-            pool_left = min(kBatch, size - b);
-            request(work, lane);
-         } else if (++tries < kSegments) {
-            seg = (seg + 1) % kSegments;
-            request(work, lane);
-         } else {
+      if (pool_left != 0 || done) return;
+      uint32_t b;
+      if (next < static_batches) {
+         b = next;
+         next += waves;
+         if (next >= static_batches) request(work, lane);     // the first dynamic batch, ahead of time
+      } else {
+         b = static_batches + __builtin_amdgcn_readlane(prefetch, 0);
+         if (b >= batches) {
             done = true;
+            return;
          }
+         request(work, lane);
       }
+      pool_base = b * kBatch;
+      pool_left = min(kBatch, n - pool_base);
    }
 };
 
@@ -323,8 +346,9 @@ struct staged_policy<P, decltype(void(P::kStaged))> {
 };
 
 // Policies that set kWindows step over window records (fetch_window,
-// advance_window) from windows(); they start at the root with s.cnt = the
-// root's split axis when the root is interior (root_count()).
+// advance_window, from windows()) once the ray queue is drained, and over
+// node records before; they start at the root with s.cnt = the root's split
+// axis when the root is interior (root_count()).
 template <class P, class = void>
 struct window_policy {
    static constexpr bool value = false;
@@ -371,8 +395,8 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
    const uint32_t lane = __lane_id();
    const Stack<kLds> st{lds_ref, lds_near, lds_cnt, reinterpret_cast<u32x4 *>(spill), threadIdx.x,
                         blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, max_stack};
-   RaySource src{n, blockIdx.x % kSegments, 0u, 0u, 0u, 0u, n == 0};
-   if (!src.done) src.request(work, lane);
+   RaySource src;
+   src.init(n, work, lane);
    Trav s;
    typename Policy::Ctx ctx;
    uint32_t mode = kIdle;
@@ -477,13 +501,20 @@ __device__ __forceinline__ void persistent_trace(const Policy &P, uint32_t n, ui
                         __any(mode != kIdle && !at_leaf);
       const uint32_t m = hold ? kHeld : mode;
       if constexpr (kWin) {
-         FetchedWin f;
-         if (m == kActive) fetch_window(s, P.windows(), P.tris(ctx), f);
-         if (m == kActive && !advance_window<kLds>(s, f, st, P.debug_pixel(ctx))) {
-            P.finish(id, ctx, s);
-            mode = kIdle;
+         // Once the ray queue is drained the wave only finishes the rays it
+         // holds, the longest ones of the launch, and each step is a wait on
+         // one dependent load: from then on it steps over window records, two
+         // levels per load.  While rays remain, node records cost less per
+         // step.
+         if (src.done && g.left == 0) {
+            FetchedWin f;
+            if (m == kActive) fetch_window(s, P.windows(), P.tris(ctx), f);
+            if (m == kActive && !advance_window<kLds>(s, f, st, P.debug_pixel(ctx))) {
+               P.finish(id, ctx, s);
+               mode = kIdle;
+            }
+            continue;
          }
-         continue;
       }
       Fetched f;
       if (!kStaged && m == kPending) {
