@@ -289,8 +289,8 @@ typedef struct ptx_tuning {
                              /* window records (a node's record and its first-       */
                              /* visited child's in one 128-byte block, 256 B per     */
                              /* interior node, built by the scene's first query      */
-                             /* with ON).  Same hits and counters; ON is slower on   */
-                             /* MI355X at every scene size measured.                 */
+                             /* with ON) once the ray queue is drained.  Same hits   */
+                             /* and counters; ON is slower on MI355X as measured.    */
 } ptx_tuning;
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t);
 int ptx_get_tuning(ptx_device *dev, ptx_tuning *t);

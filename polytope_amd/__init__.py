@@ -159,7 +159,7 @@ class Device:
         leaf_lanes: lanes a traversal wave gathers at leaves before a leaf step;
         plain_primary: render(), 0/1 primary rays traced once per pixel, 2 traced again every sample;
         ray_windows: closest-hit queries, 0/2 one node record per step, 1 window records (node + first-visited
-        child in one 128-byte block; same results, slower on MI355X)."""
+        child in one 128-byte block) once the ray queue is drained; same results, slower on MI355X."""
         t = _abi.Tuning(paths_per_wave, block_size, lds_stack, refill_lanes, blocks_per_cu, render_mode, shade_lanes,
                         leaf_lanes, plain_primary, ray_windows)
         check(load().ptx_set_tuning(self.handle, C.byref(t)))

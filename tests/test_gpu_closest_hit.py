@@ -239,10 +239,10 @@ def _window_cases():
 @pytest.mark.parametrize("lds", [8, 12])
 @pytest.mark.parametrize("case", list(_window_cases()), ids=lambda c: c[0])
 def test_gpu_window_and_node_record_forms_agree(dev, case, lds):
-    # The window form (ray_windows=1: a node's record and its first-visited
-    # child's in one 128-byte block, up to three faces per leaf step) against
-    # the one-record form and the oracle: hits, counters and frames bit for
-    # bit.  The first query on a fresh scene through the no-allocation entry
+    # The window form (ray_windows=1: once the ray queue is drained, steps over
+    # a node's record and its first-visited child's in one 128-byte block, up
+    # to three faces per leaf step) against the one-record form and the
+    # oracle: hits, counters and frames bit for bit.  The first query on a fresh scene through the no-allocation entry
     # point runs the one-record form (it never builds the windows); the device
     # entry point builds them, and the no-allocation entry point then uses them.
     name, (meshes, rays) = case
