@@ -24,7 +24,7 @@ std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jo
 
 struct RenderEngine {
    static constexpr uint32_t kDefaultPaths = 1u << 22;        // wavefront rounds (232 B per slot)
-   static constexpr uint32_t kDefaultRegenPaths = 1u << 28;   // regeneration rounds (12 B per slot)
+   static constexpr uint32_t kDefaultRegenPaths = 1u << 30;   // regeneration rounds (12 B per slot)
    static constexpr uint32_t kDefaultShadeLanes = 16;   // lanes a wave gathers before it shades
    // layout of `words`: 8 traversal work counters, the two queue counts, the
    // bounce index, the shadow-queue count
