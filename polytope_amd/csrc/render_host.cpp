@@ -228,12 +228,20 @@ std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jo
       for (uint32_t j = 0; j < jobs.size() && active.size() < max_chunks; j++)
          if (next[j] < jobs[j].desc.spp) active.push_back(j);
       if (active.empty()) break;
+      // Water-filling: the jobs with the least left go first and take what
+      // they need up to an equal share of the capacity still free, so the
+      // share they leave goes to the larger jobs and the round fills.
+      std::stable_sort(active.begin(), active.end(), [&](uint32_t a, uint32_t b) {
+         return (uint64_t)(jobs[a].desc.spp - next[a]) * jobs[a].npix <
+                (uint64_t)(jobs[b].desc.spp - next[b]) * jobs[b].npix;
+      });
       std::vector<Chunk> round;
       uint64_t used = 0;
-      const uint64_t share = cap / active.size();
-      for (uint32_t j : active) {
+      for (size_t i = 0; i < active.size(); i++) {
+         const uint32_t j = active[i];
          const uint32_t npix = jobs[j].npix;
          const uint32_t left = jobs[j].desc.spp - next[j];
+         const uint64_t share = (cap - used) / (active.size() - i);
          uint64_t k = std::max<uint64_t>(1, share / npix);
          k = std::min<uint64_t>(k, left);
          k = std::min<uint64_t>(k, (cap - used) / npix);
