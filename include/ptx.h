@@ -304,7 +304,10 @@ enum { PTX_JOB_QUEUED = 0, PTX_JOB_RUNNING = 1, PTX_JOB_DONE = 2, PTX_JOB_FAILED
 typedef struct ptx_server_config {
    uint64_t hbm_budget_bytes;    /* films + path state of admitted jobs      */
    uint32_t max_jobs_in_flight;  /* 0 = unlimited (budget only)              */
-   uint32_t paths_per_wave;      /* 0 = autotuned                            */
+   uint32_t paths_per_wave;      /* path slots per round; 0 = the device     */
+                                 /* tuning's paths_per_wave if set, else     */
+                                 /* 2^27 or what a quarter of the budget     */
+                                 /* pays for (12 B per slot), if fewer       */
 } ptx_server_config;
 
 typedef struct ptx_job_status {

@@ -263,3 +263,31 @@ def test_gpu_window_and_node_record_forms_agree(dev, case, lds):
         for x, y in zip(a, b):
             assert x.tobytes() == y.tobytes()
     assert_same_hits(*win_form, ref)
+
+
+def test_gpu_results_independent_of_grid_and_thresholds(dev):
+    # Work ids are dealt to waves by the grid's wave count (three quarters of
+    # the 64-ray batches statically, the rest from one counter), so the grid
+    # size, the refill and leaf thresholds and the LDS depth change which
+    # wave traces which ray and when; hits and the reference's counters must
+    # not change.  Ray counts around the batch and grid sizes included.
+    meshes, rays, _ = S.closest_hit_workload(6, 1 << 16, seed=11)
+    sc = Scene(dev, meshes)
+    base = gpu_query(sc, rays)
+    settings = [dict(blocks_per_cu=1), dict(blocks_per_cu=2), dict(refill_lanes=16), dict(leaf_lanes=32),
+                dict(lds_stack=8, blocks_per_cu=3)]
+    try:
+        for kw in settings:
+            dev.set_tuning(**kw)
+            print(f"case grid/thresholds {kw}", flush=True)
+            h, c, f = gpu_query(sc, rays)
+            assert h.tobytes() == base[0].tobytes(), kw
+            assert c.tobytes() == base[1].tobytes(), kw
+            assert f.tobytes() == base[2].tobytes(), kw
+            for n in (1, 63, 64, 65, 4097, 40000):
+                hs, cs, _ = gpu_query(sc, rays[:n])
+                assert hs.tobytes() == base[0][:n].tobytes(), (kw, n)
+                assert cs.tobytes() == base[1][:n].tobytes(), (kw, n)
+    finally:
+        dev.set_tuning()
+        sc.close()

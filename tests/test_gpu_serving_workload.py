@@ -1,5 +1,5 @@
 """GPU test of the render service on its serving workload: a Poisson stream of
-32 render jobs at 64 per second over displaced-icosphere scenes of 10,242 to
+32 render jobs at 512 per second over displaced-icosphere scenes of 10,242 to
 1,966,082 faces, 256x256 to 1920x1080, spp log-uniform in 4..256, depth 4..12
 (polytope_amd.scenes.serving_requests), through the continuous-batching server
 with jobs arriving while others run.
@@ -36,7 +36,7 @@ def dev():
 @pytest.mark.timeout(600)
 def test_serving_stream_equals_alone_and_oracle(dev):
     print("case serving workload: building 6 scenes (10k..2M faces)", flush=True)
-    scene_defs, reqs = S.serving_requests(32, 64.0)
+    scene_defs, reqs = S.serving_requests(32, 512.0)     # arrivals compressed so that jobs overlap
     tris = [sum(m.num_faces for m in d[0]) for d in scene_defs]
     assert min(tris) <= 10_500 and max(tris) >= 1_900_000
     assert min(r[3] for r in reqs) >= 4 and max(r[3] for r in reqs) <= 256
@@ -44,7 +44,7 @@ def test_serving_stream_equals_alone_and_oracle(dev):
     scenes = [Scene(dev, m, sky) for m, sky in scene_defs]
     srv = Server(dev, hbm_budget_bytes=64 << 30)
     try:
-        print("case serving workload: 32 requests at 64/s", flush=True)
+        print("case serving workload: 32 requests at 512/s", flush=True)
         ids = []
         t0 = time.perf_counter()
         for arrival, si, cam, spp, depth, seed in reqs:
