@@ -37,6 +37,12 @@ uint32_t render_lds_depth();
 int render_blocks_per_cu(int which);
 void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *spill, uint32_t leaf_min, uint32_t grid,
                     hipStream_t st);
+// Primary records of every job of a batch in one launch: jobs / scenes are the
+// device tables, offsets (njobs + 1 entries, device) the first pixel id of
+// each job and the total; max_stack bounds every scene's deferred stack.
+void launch_primaries(const JobDev *jobs, const DevScene *scenes, const uint32_t *offsets, uint32_t njobs,
+                      uint32_t *work, uint4 *spill, uint32_t leaf_min, uint32_t max_stack, uint32_t grid,
+                      hipStream_t st);
 // Shadow connection of the wavefront form (rounds with next-event jobs):
 // the vertex queue (48 B per entry, one entry per path slot at most), its
 // count, the per-slot next-event sums (3 floats) and the connect grid.  q null:

@@ -348,7 +348,49 @@ __device__ __forceinline__ uint32_t tri_texel(const DevScene &S, uint32_t tri, f
 // bounces, 1 = it ends here with radiance w1..w3 for src = 1), or, for a
 // textured Lambert hit, 2 | RGB texel << 8; for kinds 0 and 2: w1..3 Location,
 // w4..6 tangent1, w7..9 normal, w10..12 tangent2, w13..15 the normalised local
-// incoming direction.
+// incoming direction.  Written from the pixel's finished primary traversal s.
+__device__ __forceinline__ void write_primary_record(const DevScene &S, const JobDev &J, uint32_t pixel,
+                                                     const Trav &s) {
+   float rec[16];
+#pragma unroll
+   for (int k = 0; k < 16; k++) rec[k] = 0.f;
+   const float one[3] = {1.f, 1.f, 1.f};
+   if (s.hit == kNone) {
+      const Radiance L = miss_radiance(S, one);
+      rec[0] = __uint_as_float(1u);
+      rec[1] = L.r; rec[2] = L.g; rec[3] = L.b;
+   } else {
+      const uint32_t mesh = S.pairs[2 * (size_t)s.hit];
+      const MeshRec &M = S.meshes[mesh];
+      if (M.is_light) {
+         rec[0] = __uint_as_float(1u | (mesh << 2));
+         rec[1] = (M.emission[0] + 0.f) * 1.f;
+         rec[2] = (M.emission[1] + 0.f) * 1.f;
+         rec[3] = (M.emission[2] + 0.f) * 1.f;
+      } else if (J.max_depth == 0) {
+         rec[0] = __uint_as_float(1u | (mesh << 2));
+      } else {
+         const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
+         const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
+         float lin[3];
+         world_to_local(fr, d, lin);
+         uint32_t w0 = mesh << 2;
+         if (M.flags & kRecTextured) w0 = 2u | (tri_texel(S, s.hit, s.hu, s.hv, s.hw) & 0xFFFFFFu) << 8;
+         rec[0] = __uint_as_float(w0);
+         rec[1] = fr.lx; rec[2] = fr.ly; rec[3] = fr.lz;
+         rec[4] = fr.t1x; rec[5] = fr.t1y; rec[6] = fr.t1z;
+         rec[7] = fr.nx; rec[8] = fr.ny; rec[9] = fr.nz;
+         rec[10] = fr.t2x; rec[11] = fr.t2y; rec[12] = fr.t2z;
+         rec[13] = lin[0]; rec[14] = lin[1]; rec[15] = lin[2];
+      }
+   }
+   float4 *dst = reinterpret_cast<float4 *>(J.primary + 16 * (size_t)pixel);
+   dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
+   dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+   dst[2] = make_float4(rec[8], rec[9], rec[10], rec[11]);
+   dst[3] = make_float4(rec[12], rec[13], rec[14], rec[15]);
+}
+
 struct PrimaryPolicy {
    struct Ctx {
       uint32_t dbg;
@@ -372,44 +414,56 @@ struct PrimaryPolicy {
    __device__ __forceinline__ const float *tris(const Ctx &) const { return S.tris; }
    __device__ __forceinline__ bool debug_pixel(const Ctx &ctx) const { return ctx.dbg != 0; }
    __device__ __forceinline__ void finish(uint32_t id, const Ctx &, const Trav &s) const {
-      float rec[16];
-#pragma unroll
-      for (int k = 0; k < 16; k++) rec[k] = 0.f;
-      const float one[3] = {1.f, 1.f, 1.f};
-      if (s.hit == kNone) {
-         const Radiance L = miss_radiance(S, one);
-         rec[0] = __uint_as_float(1u);
-         rec[1] = L.r; rec[2] = L.g; rec[3] = L.b;
-      } else {
-         const uint32_t mesh = S.pairs[2 * (size_t)s.hit];
-         const MeshRec &M = S.meshes[mesh];
-         if (M.is_light) {
-            rec[0] = __uint_as_float(1u | (mesh << 2));
-            rec[1] = (M.emission[0] + 0.f) * 1.f;
-            rec[2] = (M.emission[1] + 0.f) * 1.f;
-            rec[3] = (M.emission[2] + 0.f) * 1.f;
-         } else if (J.max_depth == 0) {
-            rec[0] = __uint_as_float(1u | (mesh << 2));
-         } else {
-            const float d[3] = {s.r.dx, s.r.dy, s.r.dz};
-            const Frame fr = frame_of(S, s.hit, d, s.hu, s.hv, s.hw, s.after, mesh);
-            float lin[3];
-            world_to_local(fr, d, lin);
-            uint32_t w0 = mesh << 2;
-            if (M.flags & kRecTextured) w0 = 2u | (tri_texel(S, s.hit, s.hu, s.hv, s.hw) & 0xFFFFFFu) << 8;
-            rec[0] = __uint_as_float(w0);
-            rec[1] = fr.lx; rec[2] = fr.ly; rec[3] = fr.lz;
-            rec[4] = fr.t1x; rec[5] = fr.t1y; rec[6] = fr.t1z;
-            rec[7] = fr.nx; rec[8] = fr.ny; rec[9] = fr.nz;
-            rec[10] = fr.t2x; rec[11] = fr.t2y; rec[12] = fr.t2z;
-            rec[13] = lin[0]; rec[14] = lin[1]; rec[15] = lin[2];
-         }
+      write_primary_record(S, J, id, s);
+   }
+};
+
+// Primary rays of every job of a batch in one launch: ids are the jobs' pixels
+// back to back (offsets[j] = the first id of job j, offsets[njobs] = the
+// total), each lane finds its job by binary search and traces in that job's
+// scene.
+struct PrimariesPolicy {
+   struct Ctx {
+      uint32_t dbg, job;
+      const Node64 *nodes;
+      const float *tris;
+   };
+   const JobDev *jobs;
+   const DevScene *scenes;
+   const uint32_t *offsets;
+   uint32_t njobs;
+   __device__ __forceinline__ void load(uint32_t, Fetched &) const {}
+   __device__ __forceinline__ int start(uint32_t id, const Fetched &, Ctx &ctx, Trav &s) const {
+      uint32_t lo = 0, hi = njobs;
+      while (hi - lo > 1) {
+         const uint32_t mid = (lo + hi) / 2;
+         if (offsets[mid] <= id) lo = mid; else hi = mid;
       }
-      float4 *dst = reinterpret_cast<float4 *>(J.primary + 16 * (size_t)id);
-      dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
-      dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
-      dst[2] = make_float4(rec[8], rec[9], rec[10], rec[11]);
-      dst[3] = make_float4(rec[12], rec[13], rec[14], rec[15]);
+      const JobDev &J = jobs[lo];
+      const uint32_t p = id - offsets[lo];
+      const uint32_t x = p % J.width, y = p / J.width;
+      float o[3], d[3];
+      camera_ray(J, x, y, o, d);
+      make_ray(s.r, o[0], o[1], o[2], d[0], d[1], d[2]);
+      ctx.dbg = (x == 128 && y == 128) ? 1u : 0u;
+      ctx.job = lo;
+      const DevScene &S = scenes[J.scene];
+      ctx.nodes = S.nodes;
+      ctx.tris = S.tris;
+      return 0;
+   }
+   __device__ __forceinline__ void load2(Ctx &, Fetched &) const {}
+   __device__ __forceinline__ void start2(Ctx &, const Fetched &, Trav &) const {}
+   __device__ __forceinline__ void begin_root(Ctx &ctx, Trav &s) const {
+      const DevScene &S = scenes[jobs[ctx.job].scene];
+      begin(s, S.root_box, S.root_ref, S.root_count);
+   }
+   __device__ __forceinline__ const Node64 *nodes(const Ctx &ctx) const { return ctx.nodes; }
+   __device__ __forceinline__ const float *tris(const Ctx &ctx) const { return ctx.tris; }
+   __device__ __forceinline__ bool debug_pixel(const Ctx &ctx) const { return ctx.dbg != 0; }
+   __device__ __forceinline__ void finish(uint32_t id, const Ctx &ctx, const Trav &s) const {
+      const JobDev &J = jobs[ctx.job];
+      write_primary_record(scenes[J.scene], J, id - offsets[ctx.job], s);
    }
 };
 
@@ -417,6 +471,14 @@ template <int kLds>
 __global__ void __launch_bounds__(256) k_primary(DevScene S, JobDev J, uint32_t *work, uint4 *spill, uint32_t leaf_min) {
    const PrimaryPolicy P{nullptr, S, J};
    persistent_trace<kLds>(P, J.width * J.height, work, spill, 1u, leaf_min, S.max_stack);
+}
+
+template <int kLds>
+__global__ void __launch_bounds__(256) k_primaries(const JobDev *jobs, const DevScene *scenes, const uint32_t *offsets,
+                                                   uint32_t njobs, uint32_t *work, uint4 *spill, uint32_t leaf_min,
+                                                   uint32_t max_stack) {
+   const PrimariesPolicy P{jobs, scenes, offsets, njobs};
+   persistent_trace<kLds>(P, offsets[njobs], work, spill, 1u, leaf_min, max_stack);
 }
 
 // ---------------------------------------------------------------- path queues
@@ -1373,6 +1435,14 @@ void launch_primary(const DevScene &S, const JobDev &J, uint32_t *work, uint4 *s
                     hipStream_t st) {
    hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
    hipLaunchKernelGGL(dev::k_primary<kRenderLds>, dim3(grid), dim3(256), 0, st, S, J, work, spill, leaf_min);
+}
+
+void launch_primaries(const JobDev *jobs, const DevScene *scenes, const uint32_t *offsets, uint32_t njobs,
+                      uint32_t *work, uint4 *spill, uint32_t leaf_min, uint32_t max_stack, uint32_t grid,
+                      hipStream_t st) {
+   hipLaunchKernelGGL(dev::k_zero_words, dim3(1), dim3(64), 0, st, work, 8u, (uint32_t *)nullptr, 0u);
+   hipLaunchKernelGGL(dev::k_primaries<kRenderLds>, dim3(grid), dim3(256), 0, st, jobs, scenes, offsets, njobs, work,
+                      spill, leaf_min, max_stack);
 }
 
 void launch_round_start(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jobs,

@@ -89,7 +89,7 @@ void RenderEngine::release() {
    }
    dfree(qa); dfree(qb); dfree(hits); dfree(qs); dfree(direct); dfree(res); dfree(words); dfree(spill);
    dfree(nee_scratch);
-   dfree(d_jobs); dfree(d_scenes); dfree(d_chunks);
+   dfree(d_jobs); dfree(d_offsets); dfree(d_scenes); dfree(d_chunks);
    cap = qcap = 0;
    jobs_cap = scenes_cap = chunks_cap = 0;
    spill_entries = 0;
@@ -145,9 +145,11 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
    // a table's capacity is recorded only once its allocation succeeded
    if (njobs > jobs_cap) {
       dfree(d_jobs);
+      dfree(d_offsets);
       jobs_cap = 0;
       const uint32_t want = std::max(njobs, 64u);
       if ((rc = dalloc(&d_jobs, want))) return rc;
+      if ((rc = dalloc(&d_offsets, want + 1))) return rc;
       jobs_cap = want;
       graphs_stale = true;
    }
@@ -260,7 +262,7 @@ std::vector<std::vector<Chunk>> schedule_rounds(const std::vector<RenderJob> &jo
    return rounds;
 }
 
-int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
+int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix, bool trace_primary) {
    // stream-ordered: a job's buffers come and go between the rounds that use
    // them without waiting for the device
    J.film = nullptr;
@@ -273,8 +275,9 @@ int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix) {
       return set_error(PTX_ERR_OOM, std::string("hipMallocAsync (job film): ") + hipGetErrorString(e));
    }
    launch_zero_floats(J.film, (size_t)npix * 3, stream);
-   launch_primary(S, J, words + kWork, spill, leaf_lanes_of(tun, true), std::min<uint32_t>(ext_grid, (npix + 255) / 256),
-                  stream);
+   if (trace_primary)
+      launch_primary(S, J, words + kWork, spill, leaf_lanes_of(tun, true),
+                     std::min<uint32_t>(ext_grid, (npix + 255) / 256), stream);
    return PTX_OK;
 }
 
@@ -357,9 +360,19 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
    };
    std::vector<DevScene> sd(scenes.size());
    for (size_t i = 0; i < scenes.size(); i++) sd[i] = scenes[i]->d;
+   // every job's primary rays in one launch once the tables are up (one
+   // launch per job when the batch has more pixels than 32-bit ids cover)
+   std::vector<uint32_t> offsets(nj + 1, 0);
+   uint64_t total_pixels = 0;
+   for (uint32_t j = 0; j < nj; j++) {
+      offsets[j] = (uint32_t)total_pixels;
+      total_pixels += jobs[j].npix;
+   }
+   const bool one_launch = total_pixels < (1ull << 32);
+   offsets[nj] = (uint32_t)total_pixels;
    for (uint32_t j = 0; j < nj; j++) {
       jd[j] = job_dev(jobs[j].desc, jobs[j].scene_index);
-      if ((rc = start_job(jd[j], sd[jobs[j].scene_index], jobs[j].npix))) {
+      if ((rc = start_job(jd[j], sd[jobs[j].scene_index], jobs[j].npix, !one_launch))) {
          (void)hipStreamSynchronize(st);
          cleanup();
          return rc;
@@ -375,6 +388,8 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
          e = hipMemcpyAsync(d_scenes, sd.data(), sd.size() * sizeof(DevScene), hipMemcpyHostToDevice, st);
       if (e == hipSuccess && !all.empty())
          e = hipMemcpyAsync(d_chunks, all.data(), all.size() * sizeof(Chunk), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess && one_launch)
+         e = hipMemcpyAsync(d_offsets, offsets.data(), (nj + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st);
       if (e != hipSuccess) {
          (void)hipGetLastError();
          (void)hipStreamSynchronize(st);
@@ -382,6 +397,9 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
          return set_error(PTX_ERR_HIP, std::string("render batch tables: ") + hipGetErrorString(e));
       }
    }
+   if (one_launch)
+      launch_primaries(d_jobs, d_scenes, d_offsets, nj, words + kWork, spill, leaf_lanes_of(tun, true), max_stack,
+                       (uint32_t)std::min<uint64_t>(ext_grid, (total_pixels + 255) / 256), st);
    size_t off = 0;
    for (const auto &r : rounds) {
       uint32_t depth = 0;

@@ -48,6 +48,7 @@ struct RenderEngine {
    size_t spill_entries = 0;
    uint32_t stack_cap = 0;
    JobDev *d_jobs = nullptr;
+   uint32_t *d_offsets = nullptr;           // jobs_cap + 1 entries: first primary-ray id of each job
    uint32_t jobs_cap = 0;
    DevScene *d_scenes = nullptr;
    uint32_t scenes_cap = 0;
@@ -72,9 +73,10 @@ struct RenderEngine {
    // per-bounce graphs.
    int ensure_queues(uint32_t need);
    ShadowArgs shadow_args() const { return ShadowArgs{qs, words + kCountS, direct, connect_grid}; }
-   // Allocates the job's film and primary records, zeroes the film, traces the
-   // primary hits.  J.film / J.primary are set.
-   int start_job(JobDev &J, const DevScene &S, uint32_t npix);
+   // Allocates the job's film and primary records, zeroes the film and, with
+   // trace_primary, traces the primary hits (a batch traces every job's in
+   // one launch instead).  J.film / J.primary are set.
+   int start_job(JobDev &J, const DevScene &S, uint32_t npix, bool trace_primary = true);
    // One round over the chunks in d_round (a device copy of r); the job and
    // scene tables must hold the round's jobs, the scene table nscenes entries.
    // nee: some job of the round uses next-event estimation;
