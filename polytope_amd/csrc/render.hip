@@ -824,7 +824,7 @@ __global__ void k_finalize(float *film, uint32_t npix, uint32_t spp) {
 // count, and advances the device-resident bounce index.
 __global__ void k_bounce_prologue(uint32_t *work, uint32_t *count_out, uint32_t *bounce, uint32_t *count_shadow) {
    const uint32_t i = threadIdx.x;
-   if (i < 8) work[i] = 0u;
+   if (i < kWorkWords) work[i] = 0u;
    if (i == 8) *count_out = 0u;
    if (i == 9) *bounce += 1u;
    if (i == 10 && count_shadow) *count_shadow = 0u;

@@ -26,8 +26,9 @@ struct RenderEngine {
    static constexpr uint32_t kDefaultPaths = 1u << 22;        // wavefront rounds (232 B per slot)
    static constexpr uint32_t kDefaultRegenPaths = 1u << 30;   // regeneration rounds (12 B per slot)
    static constexpr uint32_t kDefaultShadeLanes = 16;   // lanes a wave gathers before it shades
-   // layout of `words`: 8 traversal work counters, the two queue counts, the
-   // bounce index, the shadow-queue count
+   // layout of `words`: 8 traversal work words (word 0 counts the dynamic
+   // batches handed out), the two queue counts, the bounce index, the
+   // shadow-queue count
    static constexpr uint32_t kWork = 0, kCountA = 8, kCountB = 9, kBounce = 10, kCountS = 11;
 
    ptx_device *dev = nullptr;
