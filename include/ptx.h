@@ -206,7 +206,11 @@ int ptx_bvh_build(const ptx_mesh_desc *meshes, uint32_t num_meshes, uint32_t thr
                   ptx_bvh_node *nodes, uint32_t *num_nodes, uint32_t *leaf_pairs, uint32_t *num_pairs);
 
 /* Closest hit for n rays (6 floats each: origin, direction; MinT = +inf),
- * host buffers.  counters may be NULL. */
+ * host buffers (pageable memory is fine).  counters may be NULL.  Blocks until
+ * the hits are in `hits`.  The copies go through 16 MiB of pinned host memory
+ * and device buffers sized to the largest call so far, both kept per device
+ * until ptx_device_close; up to four host threads copy while the DMA engine
+ * runs.  Calls on one device are serialised. */
 int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_hit *hits,
                           ptx_ray_counters *counters);
 /* Same, device buffers already resident in HBM, on `stream` (hipStream_t; NULL =

@@ -235,12 +235,26 @@ class Scene:
         check(load().ptx_scene_copy_bvh(self.handle, nodes.ctypes.data, pairs.ctypes.data))
         return nodes, pairs
 
-    def intersect(self, rays: np.ndarray, counters: bool = False):
-        """Closest hit of host rays (n, 6) float32 -> structured hits (and counters)."""
+    def intersect(self, rays: np.ndarray, counters: bool = False, out: Optional[tuple] = None):
+        """Closest hit of host rays (n, 6) float32 -> structured hits (and counters).
+
+        ``out``: ``(hits, counters)`` arrays to write into instead of new ones
+        (``counters`` None when not asked for), e.g. reused between calls so
+        that their pages are already mapped."""
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
-        hits = np.zeros(n, HIT_DTYPE)
-        cnt = np.zeros(n, COUNTER_DTYPE) if counters else None
+        if out is None:
+            hits = np.zeros(n, HIT_DTYPE)
+            cnt = np.zeros(n, COUNTER_DTYPE) if counters else None
+        else:
+            hits, cnt = out
+            for a, dt in ((hits, HIT_DTYPE), (cnt, COUNTER_DTYPE)):
+                if a is None:
+                    continue
+                if a.dtype != dt or a.shape != (n,) or not a.flags.c_contiguous or not a.flags.writeable:
+                    raise ValueError(f"out arrays must be writable contiguous ({n},) arrays of {dt}")
+            if hits is None or (cnt is None) == counters:
+                raise ValueError("out must hold a hits array, and a counters array exactly when counters=True")
         check(load().ptx_intersect_closest(self.handle, rays.ctypes.data, n, hits.ctypes.data,
                                            cnt.ctypes.data if cnt is not None else None))
         return (hits, cnt) if counters else hits

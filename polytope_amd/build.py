@@ -21,7 +21,7 @@ BUILD = HERE / "_build"
 LIB = HERE / "libptx_hip.so"
 ROOT = HERE.parent
 
-SOURCES = ["trace.hip", "render.hip", "capi.cpp", "scene.cpp", "bvh_build.cpp", "render_host.cpp", "server.cpp", "film_io.cpp"]
+SOURCES = ["trace.hip", "render.hip", "capi.cpp", "scene.cpp", "bvh_build.cpp", "render_host.cpp", "server.cpp", "film_io.cpp", "host_query.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",

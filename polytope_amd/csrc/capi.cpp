@@ -9,6 +9,7 @@
 #include <string>
 #include <thread>
 
+#include "host_query.h"
 #include "render_host.h"
 
 namespace ptx {
@@ -207,6 +208,7 @@ void ptx_device_close(ptx_device *dev) {
    {
       std::lock_guard<std::mutex> lock(dev->mu);
       dev->engine.reset();
+      dev->host_query.reset();
       if (dev->stream) (void)hipStreamDestroy(dev->stream);
       dev->stream = nullptr;
    }
@@ -450,31 +452,7 @@ int ptx_intersect_closest(ptx_scene *scene, const float *rays, uint32_t n, ptx_h
    ptx_device *dev = scene->dev;
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    std::lock_guard<std::mutex> lock(dev->mu);
-   hipStream_t st = dev->stream;
-   // staging buffers, released on every return path (stream-ordered)
-   struct Staging {
-      hipStream_t st;
-      void *p[3];
-      ~Staging() {
-         for (void *q : p)
-            if (q) (void)hipFreeAsync(q, st);
-      }
-   } buf{st, {nullptr, nullptr, nullptr}};
-   PTX_HIP_TRY(hipMallocAsync(&buf.p[0], size_t(n) * 24, st));
-   PTX_HIP_TRY(hipMallocAsync(&buf.p[1], size_t(n) * sizeof(ptx_hit), st));
-   if (counters) PTX_HIP_TRY(hipMallocAsync(&buf.p[2], size_t(n) * sizeof(ptx_ray_counters), st));
-   float *d_rays = static_cast<float *>(buf.p[0]);
-   ptx_hit *d_hits = static_cast<ptx_hit *>(buf.p[1]);
-   ptx_ray_counters *d_cnt = static_cast<ptx_ray_counters *>(buf.p[2]);
-   PTX_HIP_TRY(hipMemcpyAsync(d_rays, rays, size_t(n) * 24, hipMemcpyHostToDevice, st));
-   const int rc = run_closest_hit(scene, d_rays, n, d_hits, d_cnt, nullptr, st, tuning_of(dev));
-   if (rc == PTX_OK) {
-      PTX_HIP_TRY(hipMemcpyAsync(hits, d_hits, size_t(n) * sizeof(ptx_hit), hipMemcpyDeviceToHost, st));
-      if (counters)
-         PTX_HIP_TRY(hipMemcpyAsync(counters, d_cnt, size_t(n) * sizeof(ptx_ray_counters), hipMemcpyDeviceToHost, st));
-   }
-   PTX_HIP_TRY(hipStreamSynchronize(st));
-   return rc;
+   return intersect_host(scene, rays, n, hits, counters);
 }
 
 int ptx_set_tuning(ptx_device *dev, const ptx_tuning *t) {

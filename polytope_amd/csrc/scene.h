@@ -24,6 +24,7 @@ struct ptx_device;
 struct ptx_scene;
 namespace ptx {
 struct RenderEngine;
+struct HostQueryStaging;
 }
 
 namespace ptx {
@@ -92,6 +93,7 @@ struct ptx_device {
    ptx_tuning tuning{};
    std::mutex mu;                            // serialises use of the library stream
    std::unique_ptr<ptx::RenderEngine> engine;  // path queues and graphs, created on first render
+   std::unique_ptr<ptx::HostQueryStaging> host_query;  // host-buffer query staging, created on first use
    ~ptx_device();
 };
 

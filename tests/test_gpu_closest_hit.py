@@ -186,6 +186,38 @@ def test_gpu_host_and_device_entry_points_agree(dev):
     assert c_host.tobytes() == c_dev.tobytes()
 
 
+def test_gpu_host_entry_point_pipelined_copies(dev):
+    # The host-buffer entry point splits the arrays over up to eight host threads,
+    # each copying 2 MiB chunks through its own pinned slots: sizes with one
+    # chunk, exactly a chunk of rays (2 MiB / 24 B = 87,381.33), ragged shares of
+    # several chunks per thread, with and without counters, a smaller call after
+    # a larger one (staging reused), and caller-supplied output arrays.
+    meshes, rays, _ = S.closest_hit_workload(5, (1 << 21) + 4099, seed=3)
+    sc = Scene(dev, meshes)
+    h_all, c_all, _ = gpu_query(sc, rays)
+    for n, counters in ((len(rays), True), (87_381, False), (87_382, True), (700_001, False),
+                        (1, True), (len(rays) - 1, False), (174_763, True)):
+        print(f"case host pipeline n={n} counters={counters}", flush=True)
+        got = sc.intersect(rays[:n], counters=counters)
+        h = got[0] if counters else got
+        assert h.tobytes() == h_all[:n].tobytes(), n
+        if counters:
+            assert got[1].tobytes() == c_all[:n].tobytes(), n
+    # into the caller's arrays, twice (second call: pages already mapped)
+    n = 1_000_003
+    with pytest.raises(ValueError):
+        sc.intersect(rays[:n], out=(np.zeros(n * 24, np.uint8), None))     # wrong dtype
+    with pytest.raises(ValueError):
+        sc.intersect(rays[:n], counters=True, out=(np.zeros(n, HIT_DTYPE), None))
+    hits = np.zeros(n, HIT_DTYPE)
+    cnt = np.zeros(n, COUNTER_DTYPE)
+    for _ in range(2):
+        h, c = sc.intersect(rays[:n], counters=True, out=(hits, cnt))
+        assert h is hits and c is cnt
+        assert hits.tobytes() == h_all[:n].tobytes()
+        assert cnt.tobytes() == c_all[:n].tobytes()
+
+
 def test_gpu_full_size_closest_hit_sample(dev):
     # 327,680 triangles, 2^22 rays (half coherent, half incoherent): the whole batch
     # runs on the GPU; every 61st ray is checked against the oracle, and every ray
