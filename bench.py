@@ -190,7 +190,10 @@ def render_section(dev, args):
     samples = sum(c.width * c.height * spp for c, spp, _, _ in jobs)
     batch = [(scene, c, spp, d, seed) for c, spp, d, seed in jobs]
     dev.set_tuning(shade_lanes=args.shade_lanes, paths_per_wave=args.render_paths, leaf_lanes=args.leaf_lanes)
-    render_batch([(scene, jobs[0][0], 1, 8, 1)])                  # warm the engine and graphs
+    # warm-up on the same batch: the engine allocates its path slots (up to
+    # 2^30, 12 B each) on the first batch that needs them, as a long-running
+    # process does once
+    render_batch(batch)
     tb = time.perf_counter()
     films_b = render_batch(batch)
     t_batched = time.perf_counter() - tb

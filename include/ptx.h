@@ -246,7 +246,9 @@ int ptx_render(ptx_scene *scene, const ptx_job_desc *job, float *film_rgb);
  * share the GPU.  A round is one persistent path-regeneration launch (every
  * lane takes the next path slot as soon as its path ends).  The wavefront form
  * (render_mode = WAVEFRONT, a comparison form) keeps the paths in queues and
- * replays one captured hipGraph per bounce.  films[i] = W_i*H_i*3 host floats. */
+ * replays one captured hipGraph per bounce.  films[i] = W_i*H_i*3 host floats
+ * (distinct, non-overlapping buffers: they are written by several host threads
+ * at once, through the pinned staging that ptx_intersect_closest uses). */
 int ptx_render_batch(ptx_scene *const *scenes, const ptx_job_desc *jobs, uint32_t n_jobs,
                      float *const *films);
 

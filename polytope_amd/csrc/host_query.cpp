@@ -47,23 +47,35 @@ uint32_t host_workers() {
    return std::clamp<uint32_t>(c / 2, 1u, HostQueryStaging::kMaxWorkers);
 }
 
-// One array that crosses the link: elements of `elem` bytes.
-struct Span {
-   char *host;
-   char *dev;
-   size_t elem;
-};
+// Worker w's share of the spans: bytes [total * w / workers, total * (w + 1) /
+// workers) of their concatenation, cut at the spans' ends and every kSlot
+// bytes.
+std::vector<Piece> share(const std::vector<Span> &spans, uint32_t w, uint32_t workers) {
+   size_t total = 0;
+   for (const Span &a : spans) total += a.bytes;
+   const size_t lo = total / workers * w + total % workers * w / workers;
+   const size_t hi = total / workers * (w + 1) + total % workers * (w + 1) / workers;
+   std::vector<Piece> out;
+   size_t base = 0;
+   for (const Span &a : spans) {
+      const size_t b0 = std::max(lo, base), b1 = std::min(hi, base + a.bytes);
+      for (size_t off = b0; off < b1; off += kSlot)
+         out.push_back(Piece{a.host + (off - base), a.dev + (off - base), std::min(kSlot, b1 - off)});
+      base += a.bytes;
+   }
+   return out;
+}
 
-// State one call's workers share.  Phase 0: rays to the device; the main
-// thread launches the query once every worker has arrived; phase 1: results
-// back; -1: a step failed, the rest is skipped.
+// State one call's workers share.  Phase 0: host spans to the device; the
+// calling thread runs `between` (the launch) once every worker has queued
+// its copies; phase 1: device spans back to the host; -1: a step failed, the
+// rest is skipped.
 struct HostCall {
    HostQueryStaging *st;
    hipStream_t stream;
    int ordinal;
-   uint32_t n, workers;
-   Span in;
-   std::vector<Span> out;
+   uint32_t workers = 1;
+   std::vector<Span> in, out;
    std::mutex mu;
    std::condition_variable cv;
    uint32_t arrived = 0;
@@ -80,53 +92,37 @@ struct HostCall {
       fail(std::string(what) + ": " + hipGetErrorString(e));
       return false;
    }
-   // worker w's elements: [lo, hi)
-   void range(uint32_t w, size_t &lo, size_t &hi) const {
-      lo = size_t(n) * w / workers;
-      hi = size_t(n) * (w + 1) / workers;
-   }
    char *slot(uint32_t w, uint32_t k) const { return st->pinned + (2 * size_t(w) + (k & 1)) * kSlot; }
    hipEvent_t event(uint32_t w, uint32_t k) const { return st->events[2 * w + (k & 1)]; }
 
-   // Rays of worker w: copy a chunk into a slot, then queue its DMA; a slot
-   // is refilled once the DMA that read it has finished.
+   // Worker w's share going up: copy a piece into a slot, then queue its DMA;
+   // a slot is refilled once the DMA that read it has finished.
    bool send(uint32_t w) {
-      size_t lo, hi;
-      range(w, lo, hi);
-      const size_t b0 = lo * in.elem, b1 = hi * in.elem;
-      uint32_t k = 0;
-      for (size_t off = b0; off < b1; off += kSlot, k++) {
-         const size_t len = std::min(kSlot, b1 - off);
+      const std::vector<Piece> ps = share(in, w, workers);
+      for (uint32_t k = 0; k < ps.size(); k++) {
          if (k >= 2 && !ok(hipEventSynchronize(event(w, k)), "hipEventSynchronize")) return false;
-         std::memcpy(slot(w, k), in.host + off, len);
-         if (!ok(hipMemcpyAsync(in.dev + off, slot(w, k), len, hipMemcpyHostToDevice, stream), "hipMemcpyAsync"))
+         std::memcpy(slot(w, k), ps[k].host, ps[k].len);
+         if (!ok(hipMemcpyAsync(ps[k].dev, slot(w, k), ps[k].len, hipMemcpyHostToDevice, stream), "hipMemcpyAsync"))
             return false;
          if (!ok(hipEventRecord(event(w, k), stream), "hipEventRecord")) return false;
       }
       return true;
    }
 
-   // Results of worker w: the DMA of chunk k + 1 is queued before chunk k is
-   // copied out of its slot.
+   // Worker w's share coming back: the DMA of piece k + 1 is queued before
+   // piece k is copied out of its slot.
    bool receive(uint32_t w) {
-      size_t lo, hi;
-      range(w, lo, hi);
-      for (const Span &a : out) {
-         const size_t b0 = lo * a.elem, b1 = hi * a.elem;
-         if (b0 == b1) continue;
-         const uint32_t chunks = uint32_t((b1 - b0 + kSlot - 1) / kSlot);
-         auto queue = [&](uint32_t k) {
-            const size_t off = b0 + size_t(k) * kSlot, len = std::min(kSlot, b1 - off);
-            return ok(hipMemcpyAsync(slot(w, k), a.dev + off, len, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync") &&
-                   ok(hipEventRecord(event(w, k), stream), "hipEventRecord");
-         };
-         if (!queue(0)) return false;
-         for (uint32_t k = 0; k < chunks; k++) {
-            if (k + 1 < chunks && !queue(k + 1)) return false;
-            if (!ok(hipEventSynchronize(event(w, k)), "hipEventSynchronize")) return false;
-            const size_t off = b0 + size_t(k) * kSlot;
-            std::memcpy(a.host + off, slot(w, k), std::min(kSlot, b1 - off));
-         }
+      const std::vector<Piece> ps = share(out, w, workers);
+      auto queue = [&](uint32_t k) {
+         return ok(hipMemcpyAsync(slot(w, k), ps[k].dev, ps[k].len, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync") &&
+                ok(hipEventRecord(event(w, k), stream), "hipEventRecord");
+      };
+      if (ps.empty()) return true;
+      if (!queue(0)) return false;
+      for (uint32_t k = 0; k < ps.size(); k++) {
+         if (k + 1 < ps.size() && !queue(k + 1)) return false;
+         if (!ok(hipEventSynchronize(event(w, k)), "hipEventSynchronize")) return false;
+         std::memcpy(ps[k].host, slot(w, k), ps[k].len);
       }
       return true;
    }
@@ -141,6 +137,53 @@ struct HostCall {
       if (phase < 0) return;
       lock.unlock();
       receive(w);
+   }
+
+   // The whole call on the calling thread and up to `workers` threads.
+   template <class F>
+   int run(F between) {
+      size_t in_bytes = 0, out_bytes = 0;
+      for (const Span &a : in) in_bytes += a.bytes;
+      for (const Span &a : out) out_bytes += a.bytes;
+      const size_t pieces = (std::max(in_bytes, out_bytes) + kSlot - 1) / kSlot;
+      workers = uint32_t(std::min<size_t>(host_workers(), std::max<size_t>(1, pieces)));
+      int rc = PTX_OK;
+      if (workers == 1) {
+         if (send(0)) {
+            rc = between();
+            if (rc == PTX_OK) receive(0);
+         }
+      } else {
+         std::vector<std::thread> ts;
+         ts.reserve(workers);
+         try {
+            for (uint32_t w = 0; w < workers; w++) ts.emplace_back([this, w] { worker(w); });
+         } catch (const std::exception &) {
+            fail("could not start the host copy threads");
+         }
+         std::unique_lock<std::mutex> lock(mu);
+         // threads that did not start never arrive: count them as arrived
+         arrived += workers - uint32_t(ts.size());
+         cv.wait(lock, [&] { return arrived == workers; });
+         if (error.empty()) {
+            lock.unlock();
+            rc = between();
+            lock.lock();
+         }
+         phase = (rc == PTX_OK && error.empty()) ? 1 : -1;
+         cv.notify_all();
+         lock.unlock();
+         for (std::thread &th : ts) th.join();
+      }
+      // every queued copy has finished before the slots serve another call
+      const hipError_t se = hipStreamSynchronize(stream);
+      if (rc != PTX_OK) return rc;
+      if (!error.empty()) return set_error(PTX_ERR_HIP, error);
+      if (se != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(PTX_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+      }
+      return PTX_OK;
    }
 };
 
@@ -183,55 +226,26 @@ int intersect_host(ptx_scene *s, const float *rays, uint32_t n, ptx_hit *hits, p
    call.st = &st;
    call.stream = dev->stream;
    call.ordinal = dev->ordinal;
-   call.n = n;
-   call.in = Span{reinterpret_cast<char *>(const_cast<float *>(rays)), st.d_buf, 24};
    char *d_hits = st.d_buf + b_rays, *d_cnt = counters ? d_hits + b_hits : nullptr;
-   call.out.push_back(Span{reinterpret_cast<char *>(hits), d_hits, sizeof(ptx_hit)});
-   if (counters) call.out.push_back(Span{reinterpret_cast<char *>(counters), d_cnt, sizeof(ptx_ray_counters)});
-   const size_t chunks = (size_t(n) * 24 + kSlot - 1) / kSlot;
-   call.workers = uint32_t(std::min<size_t>(host_workers(), std::max<size_t>(1, chunks)));
+   call.in.push_back(Span{reinterpret_cast<char *>(const_cast<float *>(rays)), st.d_buf, size_t(n) * 24});
+   call.out.push_back(Span{reinterpret_cast<char *>(hits), d_hits, size_t(n) * sizeof(ptx_hit)});
+   if (counters)
+      call.out.push_back(Span{reinterpret_cast<char *>(counters), d_cnt, size_t(n) * sizeof(ptx_ray_counters)});
    const ptx_tuning t = tuning_of(dev);
-   auto launch = [&] {
+   return call.run([&] {
       return run_closest_hit(s, reinterpret_cast<const float *>(st.d_buf), n, reinterpret_cast<ptx_hit *>(d_hits),
                              reinterpret_cast<ptx_ray_counters *>(d_cnt), nullptr, call.stream, t);
-   };
-   int rc = PTX_OK;
-   if (call.workers == 1) {
-      if (call.send(0)) {
-         rc = launch();
-         if (rc == PTX_OK) call.receive(0);
-      }
-   } else {
-      std::vector<std::thread> ts;
-      ts.reserve(call.workers);
-      try {
-         for (uint32_t w = 0; w < call.workers; w++) ts.emplace_back([&call, w] { call.worker(w); });
-      } catch (const std::exception &) {
-         call.fail("could not start the host copy threads");
-      }
-      std::unique_lock<std::mutex> lock(call.mu);
-      // threads that did not start never arrive: count them as arrived
-      call.arrived += call.workers - uint32_t(ts.size());
-      call.cv.wait(lock, [&] { return call.arrived == call.workers; });
-      if (call.error.empty()) {
-         lock.unlock();
-         rc = launch();
-         lock.lock();
-      }
-      call.phase = (rc == PTX_OK && call.error.empty()) ? 1 : -1;
-      call.cv.notify_all();
-      lock.unlock();
-      for (std::thread &th : ts) th.join();
-   }
-   // every queued copy has finished before the slots serve another call
-   const hipError_t se = hipStreamSynchronize(call.stream);
-   if (rc != PTX_OK) return rc;
-   if (!call.error.empty()) return set_error(PTX_ERR_HIP, call.error);
-   if (se != hipSuccess) {
-      (void)hipGetLastError();
-      return set_error(PTX_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
-   }
-   return PTX_OK;
+   });
+}
+
+int download_host(ptx_device *dev, hipStream_t stream, const std::vector<Span> &spans) {
+   if (int rc = ensure_staging(dev, 0)) return rc;
+   HostCall call;
+   call.st = dev->host_query.get();
+   call.stream = stream;
+   call.ordinal = dev->ordinal;
+   call.out = spans;
+   return call.run([] { return int(PTX_OK); });
 }
 
 }  // namespace ptx

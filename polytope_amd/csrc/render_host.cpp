@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "render_host.h"
+#include "host_query.h"
 
 namespace ptx {
 
@@ -417,17 +418,21 @@ int RenderEngine::run_batch(const std::vector<RenderJob> &jobs, const std::vecto
       }
       off += r.size();
    }
+   // films back through the pinned staging, several host threads at once
+   std::vector<Span> films;
    for (uint32_t j = 0; j < nj; j++) {
-      if ((rc = finish_job(jd[j], jobs[j].npix, jobs[j].desc.spp, films_host[j]))) {
+      if ((rc = finish_job(jd[j], jobs[j].npix, jobs[j].desc.spp, nullptr))) {
          (void)hipStreamSynchronize(st);
          cleanup();
          return rc;
       }
+      if (films_host[j])
+         films.push_back(Span{reinterpret_cast<char *>(films_host[j]), reinterpret_cast<char *>(jd[j].film),
+                              (size_t)jobs[j].npix * 3 * sizeof(float)});
    }
-   const hipError_t se = hipStreamSynchronize(st);
+   rc = download_host(dev, st, films);
    cleanup();
-   if (se != hipSuccess) return set_error(PTX_ERR_HIP, std::string("render batch: ") + hipGetErrorString(se));
-   return PTX_OK;
+   return rc;
 }
 
 int RenderEngine::run_plain(const ptx_scene *scene, const ptx_job_desc &desc, float *film_host) {
@@ -456,13 +461,12 @@ int RenderEngine::run_plain(const ptx_scene *scene, const ptx_job_desc &desc, fl
          std::min<uint32_t>((desc.flags & PTX_JOB_NEE) ? mega_nee_grid : mega_grid, (npix + 255) / 256);
    for (uint32_t s = 0; s < desc.spp; s++) launch_mega(scene->d, J, s, spill, scene->has_texture, cached, grid, st);
    launch_finalize(film, npix, desc.spp, st);
-   hipError_t e = hipMemcpyAsync(film_host, film, (size_t)npix * 3 * sizeof(float), hipMemcpyDeviceToHost, st);
-   if (e == hipSuccess) e = hipStreamSynchronize(st);
-   if (e != hipSuccess) (void)hipGetLastError();
+   rc = download_host(dev, st,
+                      {Span{reinterpret_cast<char *>(film_host), reinterpret_cast<char *>(film),
+                            (size_t)npix * 3 * sizeof(float)}});
    dfree(film);
    dfree(primary);
-   if (e != hipSuccess) return set_error(PTX_ERR_HIP, std::string("render: ") + hipGetErrorString(e));
-   return PTX_OK;
+   return rc;
 }
 
 RenderEngine *engine_of(ptx_device *dev) {
@@ -653,10 +657,12 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
       job.spp = 32;
       job.max_depth = 4;
       job.seed = 1;
-      std::vector<float> film((size_t)256 * 256 * 3);
+      // one film per job: the films come back on several host threads at once
+      std::vector<float> film((size_t)4 * 256 * 256 * 3);
       std::vector<ptx::RenderJob> rj(4);
       for (auto &r : rj) { r.desc = job; r.scene_index = 0; r.npix = 256 * 256; }
-      std::vector<float *> films(4, film.data());
+      std::vector<float *> films(4);
+      for (size_t i = 0; i < films.size(); i++) films[i] = film.data() + i * 256 * 256 * 3;
       double best_s = 1e30;
       uint32_t best_paths = best.paths_per_wave, best_mode = best.render_mode;
       std::lock_guard<std::mutex> lock(dev->mu);

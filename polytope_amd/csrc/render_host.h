@@ -84,6 +84,8 @@ struct RenderEngine {
    // glossy / textured: some scene of the round has glossy / textured meshes.
    int round(const Chunk *d_round, const std::vector<Chunk> &r, uint32_t depth, uint32_t nscenes, bool nee,
              bool glossy, bool textured);
+   // Queues the film's finalisation and, when film_host is not NULL (the
+   // server's pinned films), its copy there.
    int finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film_host);
    void free_job(JobDev &J);
    uint32_t capacity_for(uint64_t total_paths, uint32_t max_npix, bool regen) const;
