@@ -185,6 +185,11 @@ typedef struct ptx_ray_counters {
 const char *ptx_last_error(void);
 const char *ptx_version(void);
 
+/* Opens a gfx950 device.  The handle owns a stream-ordered memory pool that
+ * keeps freed blocks for the library's next allocations, and the render
+ * engine keeps finished jobs' films and primary records (up to 4 GB) for
+ * later jobs; both are released on an allocation failure of the library and
+ * when the handle ends. */
 int ptx_device_open(int ordinal, ptx_device **out);
 /* Ends the handle: no new scene, query, render or server on it.  Scenes and
  * servers made on it stay valid until destroyed (they keep the device state

@@ -55,6 +55,14 @@ void device_release(ptx_device *dev) {
    if (dev->refs.fetch_sub(1) == 1) delete dev;
 }
 
+hipError_t pool_malloc(const ptx_device *dev, void **p, size_t bytes, hipStream_t stream) {
+   return dev->pool ? hipMallocFromPoolAsync(p, bytes, dev->pool, stream) : hipMallocAsync(p, bytes, stream);
+}
+
+void pool_trim(const ptx_device *dev) {
+   if (dev->pool && hipMemPoolTrimTo(dev->pool, 0) != hipSuccess) (void)hipGetLastError();
+}
+
 int device_usable(const ptx_device *dev) {
    if (!dev) return set_error(PTX_ERR_INVALID, "NULL device");
    if (dev->closed.load()) return set_error(PTX_ERR_INVALID, "the device has been closed");
@@ -139,7 +147,7 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
    const uint32_t needed_blocks = (n + 255) / 256;
    grid = std::max(1u, std::min(grid, needed_blocks));
    void *scratch = nullptr;
-   PTX_HIP_TRY(hipMallocAsync(&scratch, closest_hit_scratch_bytes(s, grid, lds), stream));
+   PTX_HIP_TRY(pool_malloc(dev, &scratch, closest_hit_scratch_bytes(s, grid, lds), stream));
    uint32_t *work = static_cast<uint32_t *>(scratch);
    uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(scratch) + 256);
    const uint32_t refill = t.refill_lanes ? std::min(64u, t.refill_lanes) : 1u;
@@ -154,7 +162,14 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
 
 }  // namespace ptx
 
-ptx_device::~ptx_device() = default;
+ptx_device::~ptx_device() {
+   // every allocation from the pool has been freed by now (scenes and servers
+   // keep the device alive until they are gone)
+   if (pool) {
+      (void)hipSetDevice(ordinal);
+      (void)hipMemPoolDestroy(pool);
+   }
+}
 
 ptx_scene::~ptx_scene() {
    if ((!allocations.empty() || windows) && dev) (void)hipSetDevice(dev->ordinal);
@@ -198,6 +213,23 @@ int ptx_device_open(int ordinal, ptx_device **out) {
    // set-up of the no-allocation entry point: this device's copy of its static
    // scratch (a failure here only disables that entry point)
    if (noalloc_scratch_address(ordinal, &d->noalloc_scratch) != 0) d->noalloc_scratch = nullptr;
+   // the library's own pool keeps freed blocks (a failure here leaves the
+   // default pool in use)
+   hipMemPoolProps props{};
+   props.allocType = hipMemAllocationTypePinned;
+   props.handleTypes = hipMemHandleTypeNone;
+   props.location.type = hipMemLocationTypeDevice;
+   props.location.id = ordinal;
+   if (hipMemPoolCreate(&d->pool, &props) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      if (hipMemPoolSetAttribute(d->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+         (void)hipMemPoolDestroy(d->pool);
+         d->pool = nullptr;
+      }
+   } else {
+      d->pool = nullptr;
+   }
+   (void)hipGetLastError();
    *out = d;
    return PTX_OK;
 }

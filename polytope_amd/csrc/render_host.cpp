@@ -27,11 +27,21 @@ namespace ptx {
 
 namespace {
 
+// Device buffer of n T for the engine; when the allocation fails, the
+// engine's spare job buffers and the device pool's unused blocks go back to
+// the driver and it is tried once more.
 template <class T>
-int dalloc(T **p, size_t n) {
+int dalloc(RenderEngine *eng, T **p, size_t n) {
    *p = nullptr;
    if (n == 0) return PTX_OK;
-   const hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+   hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+   if (e != hipSuccess && eng->dev && (eng->dev->pool || !eng->spares.empty())) {
+      (void)hipGetLastError();
+      eng->drop_spares();
+      (void)hipStreamSynchronize(eng->stream);
+      pool_trim(eng->dev);
+      e = hipMalloc((void **)p, n * sizeof(T));
+   }
    if (e != hipSuccess) {
       (void)hipGetLastError();      // the caller's own later HIP calls must not see this failure
       *p = nullptr;
@@ -83,7 +93,55 @@ size_t job_device_bytes(const ptx_job_desc &j) {
 
 RenderEngine::~RenderEngine() { release(); }
 
+hipError_t RenderEngine::take_buffer(void **p, size_t bytes) {
+   size_t best = spares.size();
+   for (size_t i = 0; i < spares.size(); i++)
+      if (spares[i].bytes >= bytes && spares[i].bytes / 2 <= bytes &&
+          (best == spares.size() || spares[i].bytes < spares[best].bytes))
+         best = i;
+   if (best < spares.size()) {
+      *p = spares[best].p;
+      lent.push_back(spares[best]);
+      spare_bytes -= spares[best].bytes;
+      spares.erase(spares.begin() + (ptrdiff_t)best);
+      return hipSuccess;
+   }
+   hipError_t e = pool_malloc(dev, p, bytes, stream);
+   if (e != hipSuccess && !spares.empty()) {
+      (void)hipGetLastError();
+      drop_spares();
+      e = pool_malloc(dev, p, bytes, stream);
+   }
+   if (e == hipSuccess) lent.push_back(Spare{*p, bytes});
+   return e;
+}
+
+void RenderEngine::give_buffer(void *p) {
+   if (!p) return;
+   size_t bytes = 0;
+   for (size_t i = 0; i < lent.size(); i++)
+      if (lent[i].p == p) {
+         bytes = lent[i].bytes;
+         lent[i] = lent.back();
+         lent.pop_back();
+         break;
+      }
+   if (bytes && spare_bytes + bytes <= spare_cap) {
+      spares.push_back(Spare{p, bytes});
+      spare_bytes += bytes;
+   } else {
+      (void)hipFreeAsync(p, stream);
+   }
+}
+
+void RenderEngine::drop_spares() {
+   for (const Spare &b : spares) (void)hipFreeAsync(b.p, stream);
+   spares.clear();
+   spare_bytes = 0;
+}
+
 void RenderEngine::release() {
+   drop_spares();
    for (hipGraphExec_t *g : {&g_ab, &g_ba, &g_ab_nee, &g_ba_nee}) {
       if (*g) (void)hipGraphExecDestroy(*g);
       *g = nullptr;
@@ -122,7 +180,7 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
    int rc;
    if (paths > cap) {
       dfree(res);
-      if ((rc = dalloc(&res, (size_t)paths * 3))) {
+      if ((rc = dalloc(this, &res, (size_t)paths * 3))) {
          release();
          return rc;
       }
@@ -130,12 +188,12 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       graphs_stale = true;
    }
    if (!words) {
-      if ((rc = dalloc(&words, 64))) return rc;
+      if ((rc = dalloc(this, &words, 64))) return rc;
       graphs_stale = true;
    }
    if (need_spill > spill_entries) {
       dfree(spill);
-      if ((rc = dalloc(&spill, need_spill))) { release(); return rc; }
+      if ((rc = dalloc(this, &spill, need_spill))) { release(); return rc; }
       spill_entries = need_spill;
       graphs_stale = true;
    }
@@ -149,8 +207,8 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       dfree(d_offsets);
       jobs_cap = 0;
       const uint32_t want = std::max(njobs, 64u);
-      if ((rc = dalloc(&d_jobs, want))) return rc;
-      if ((rc = dalloc(&d_offsets, want + 1))) return rc;
+      if ((rc = dalloc(this, &d_jobs, want))) return rc;
+      if ((rc = dalloc(this, &d_offsets, want + 1))) return rc;
       jobs_cap = want;
       graphs_stale = true;
    }
@@ -158,7 +216,7 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       dfree(d_scenes);
       scenes_cap = 0;
       const uint32_t want = std::max(nscenes, 16u);
-      if ((rc = dalloc(&d_scenes, want))) return rc;
+      if ((rc = dalloc(this, &d_scenes, want))) return rc;
       scenes_cap = want;
       graphs_stale = true;
    }
@@ -166,7 +224,7 @@ int RenderEngine::ensure(uint32_t paths, uint32_t max_stack, uint32_t njobs, uin
       dfree(d_chunks);
       chunks_cap = 0;
       const uint32_t want = std::max(nchunks, 1024u);
-      if ((rc = dalloc(&d_chunks, want))) return rc;
+      if ((rc = dalloc(this, &d_chunks, want))) return rc;
       chunks_cap = want;
    }
    return PTX_OK;
@@ -177,9 +235,9 @@ int RenderEngine::ensure_queues(uint32_t need) {
    if (qcap < need) {
       dfree(qa); dfree(qb); dfree(hits); dfree(qs); dfree(direct);
       qcap = 0;
-      if ((rc = dalloc(&qa, (size_t)need * 16)) || (rc = dalloc(&qb, (size_t)need * 16)) ||
-          (rc = dalloc(&hits, (size_t)need * 8)) || (rc = dalloc(&qs, (size_t)need * 3)) ||
-          (rc = dalloc(&direct, (size_t)need * 3))) {
+      if ((rc = dalloc(this, &qa, (size_t)need * 16)) || (rc = dalloc(this, &qb, (size_t)need * 16)) ||
+          (rc = dalloc(this, &hits, (size_t)need * 8)) || (rc = dalloc(this, &qs, (size_t)need * 3)) ||
+          (rc = dalloc(this, &direct, (size_t)need * 3))) {
          dfree(qa); dfree(qb); dfree(hits); dfree(qs); dfree(direct);
          return rc;
       }
@@ -268,12 +326,12 @@ int RenderEngine::start_job(JobDev &J, const DevScene &S, uint32_t npix, bool tr
    // them without waiting for the device
    J.film = nullptr;
    J.primary = nullptr;
-   hipError_t e = hipMallocAsync((void **)&J.film, (size_t)npix * 3 * sizeof(float), stream);
-   if (e == hipSuccess) e = hipMallocAsync((void **)&J.primary, (size_t)npix * 16 * sizeof(uint32_t), stream);
+   hipError_t e = take_buffer((void **)&J.film, (size_t)npix * 3 * sizeof(float));
+   if (e == hipSuccess) e = take_buffer((void **)&J.primary, (size_t)npix * 16 * sizeof(uint32_t));
    if (e != hipSuccess) {
       (void)hipGetLastError();
       free_job(J);
-      return set_error(PTX_ERR_OOM, std::string("hipMallocAsync (job film): ") + hipGetErrorString(e));
+      return set_error(PTX_ERR_OOM, std::string("job film and primary records: ") + hipGetErrorString(e));
    }
    launch_zero_floats(J.film, (size_t)npix * 3, stream);
    if (trace_primary)
@@ -287,7 +345,7 @@ int RenderEngine::round(const Chunk *d_round, const std::vector<Chunk> &r, uint3
    uint32_t nslots = 0;
    for (const Chunk &c : r) nslots = std::max(nslots, c.base + c.k * c.npix);
    if (nee && regen_round((uint32_t)r.size(), nscenes) && !nee_scratch) {
-      const int rc = dalloc(&nee_scratch, (size_t)paths_grid * 256 * 3);
+      const int rc = dalloc(this, &nee_scratch, (size_t)paths_grid * 256 * 3);
       if (rc) return rc;
    }
    if (regen_round((uint32_t)r.size(), nscenes)) {
@@ -324,8 +382,8 @@ int RenderEngine::finish_job(JobDev &J, uint32_t npix, uint32_t spp, float *film
 }
 
 void RenderEngine::free_job(JobDev &J) {
-   if (J.film) (void)hipFreeAsync(J.film, stream);
-   if (J.primary) (void)hipFreeAsync(J.primary, stream);
+   give_buffer(J.film);
+   give_buffer(J.primary);
    J.film = nullptr;
    J.primary = nullptr;
 }
@@ -445,10 +503,12 @@ int RenderEngine::run_plain(const ptx_scene *scene, const ptx_job_desc &desc, fl
    const bool cached = tun.plain_primary != PTX_PLAIN_RETRACE;
    float *film = nullptr;
    uint32_t *primary = nullptr;
-   if ((rc = dalloc(&film, (size_t)npix * 3))) return rc;
-   if (cached && (rc = dalloc(&primary, (size_t)npix * 16))) {
-      dfree(film);
-      return rc;
+   hipError_t e = take_buffer((void **)&film, (size_t)npix * 3 * sizeof(float));
+   if (e == hipSuccess && cached) e = take_buffer((void **)&primary, (size_t)npix * 16 * sizeof(uint32_t));
+   if (e != hipSuccess) {
+      (void)hipGetLastError();
+      give_buffer(film);
+      return set_error(PTX_ERR_OOM, std::string("plain film and primary records: ") + hipGetErrorString(e));
    }
    JobDev J = job_dev(desc, 0);
    J.film = film;
@@ -464,8 +524,8 @@ int RenderEngine::run_plain(const ptx_scene *scene, const ptx_job_desc &desc, fl
    rc = download_host(dev, st,
                       {Span{reinterpret_cast<char *>(film_host), reinterpret_cast<char *>(film),
                             (size_t)npix * 3 * sizeof(float)}});
-   dfree(film);
-   dfree(primary);
+   give_buffer(film);
+   give_buffer(primary);
    return rc;
 }
 

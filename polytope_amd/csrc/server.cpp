@@ -438,6 +438,7 @@ int ptx_server_create(ptx_device *dev, const ptx_server_config *cfg, ptx_server 
    PTX_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
    s->eng.dev = dev;
    s->eng.stream = s->stream;
+   s->eng.spare_cap = 0;        // job buffers count against the server's budget only while jobs hold them
    // the server keeps the tuning it was created with: a later ptx_set_tuning
    // changes neither its round form nor the pool its budget was charged for
    s->eng.tun = tuning_of(dev);

@@ -249,3 +249,26 @@ def test_render_batch_more_scenes_than_lds_table(dev):
         assert np.array_equal(alone.view(np.uint32), grouped[i].view(np.uint32)), i
     for s in scenes:
         s.close()
+
+
+def test_job_buffers_reused_across_calls(dev):
+    # A finished job's film and primary records stay with the engine as spares
+    # and a later job of at most that size takes them (the film is zeroed, the
+    # primary records rewritten): the same job gives the same film first, after
+    # a larger and a smaller job left their buffers, and in the plain path.
+    meshes, cam_s, sky = _lambert_scene(40, 30)
+    sc = Scene(dev, meshes, sky)
+    cam_l = S.Camera(cam_s.camera_to_world, 60.0, 96, 72)
+    cam_m = S.Camera(cam_s.camera_to_world, 60.0, 80, 60)     # fits the larger job's buffers (at most 2x)
+    print("case job buffers reused", flush=True)
+    (first,) = render_batch([(sc, cam_s, 3, 4, 17)])
+    render_batch([(sc, cam_l, 2, 3, 18), (sc, cam_s, 1, 2, 19)])      # larger buffers, then a spare of this size
+    (again,) = render_batch([(sc, cam_s, 3, 4, 17)])
+    (mid,) = render_batch([(sc, cam_m, 2, 5, 20)])                     # takes the larger job's spares
+    plain = render(sc, cam_s, 3, 4, 17)
+    (last,) = render_batch([(sc, cam_m, 2, 5, 20), (sc, cam_s, 3, 4, 17)])[1:]
+    for f in (again, plain, last):
+        assert np.array_equal(f.view(np.uint32), first.view(np.uint32))
+    ref = O.OracleScene(meshes, sky).render(cam_m, 2, 5, 20)
+    assert np.array_equal(mid.view(np.uint32), ref.view(np.uint32))
+    sc.close()
