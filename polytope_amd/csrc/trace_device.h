@@ -94,7 +94,13 @@ __device__ __forceinline__ void make_ray(Ray &r, float ox, float oy, float oz, f
 //   inside || (near <= far && near <= m),
 // because its far bound is the same fold started from m instead of +inf and
 // its running near/far are monotone, so its early exits agree with the final
-// comparison.
+// comparison.  The per-axis swap keeps the reference's selections (with a NaN
+// slab value, from 0 * inf, it decides which value goes on).  The folds are
+// fmaxf/fminf, which the compiler merges into three-input max/min: the
+// reference's `near = tn > near ? tn : near` keeps near when tn is NaN, and so
+// does fmaxf because near starts at 0 and is never NaN (likewise far from
+// +inf).  The two can differ only in the sign of a zero result, and near and
+// far are only ever compared, where -0 == +0.
 __device__ __forceinline__ void slab(float b0, float b1, float b2, float b3, float b4, float b5,
                                      const Ray &r, bool &inside, float &near, float &far) {
    inside = r.ox > b0 && r.ox < b3 && r.oy > b1 && r.oy < b4 && r.oz > b2 && r.oz < b5;
@@ -103,13 +109,13 @@ __device__ __forceinline__ void slab(float b0, float b1, float b2, float b3, flo
    float tn, tf, sw;
    tn = (b0 - r.ox) * r.ix; tf = (b3 - r.ox) * r.ix;
    sw = tn; tn = tn > tf ? tf : tn; tf = sw > tf ? sw : tf; tf *= kSlabScale;
-   near = tn > near ? tn : near; far = tf < far ? tf : far;
+   near = fmaxf(near, tn); far = fminf(far, tf);
    tn = (b1 - r.oy) * r.iy; tf = (b4 - r.oy) * r.iy;
    sw = tn; tn = tn > tf ? tf : tn; tf = sw > tf ? sw : tf; tf *= kSlabScale;
-   near = tn > near ? tn : near; far = tf < far ? tf : far;
+   near = fmaxf(near, tn); far = fminf(far, tf);
    tn = (b2 - r.oz) * r.iz; tf = (b5 - r.oz) * r.iz;
    sw = tn; tn = tn > tf ? tf : tn; tf = sw > tf ? sw : tf; tf *= kSlabScale;
-   near = tn > near ? tn : near; far = tf < far ? tf : far;
+   near = fmaxf(near, tn); far = fminf(far, tf);
 }
 
 __device__ __forceinline__ float max_component(float x, float y, float z) {   // mesh.cpp:149-155
