@@ -88,22 +88,28 @@ __device__ __forceinline__ void make_ray(Ray &r, float ox, float oy, float oz, f
    r.Sz = 1.f / pdz;
 }
 
-// BoundingBox::Hits split in two (BoundingBox.cpp:12-85): `inside` and the
-// slab interval [near, far] folded from near = 0 and far = +inf with the
-// reference's selections.  The reference result for MinT m is
+// BoundingBox::Hits (BoundingBox.cpp:12-85) as the slab interval [near, far]
+// folded from near = 0 and far = +inf with the reference's selections.  The
+// reference result for MinT m is
 //   inside || (near <= far && near <= m),
 // because its far bound is the same fold started from m instead of +inf and
 // its running near/far are monotone, so its early exits agree with the final
-// comparison.  The per-axis swap keeps the reference's selections (with a NaN
-// slab value, from 0 * inf, it decides which value goes on).  The folds are
-// fmaxf/fminf, which the compiler merges into three-input max/min: the
-// reference's `near = tn > near ? tn : near` keeps near when tn is NaN, and so
-// does fmaxf because near starts at 0 and is never NaN (likewise far from
-// +inf).  The two can differ only in the sign of a zero result, and near and
-// far are only ever compared, where -0 == +0.
+// comparison.  Its `inside` term (origin strictly inside the box) never
+// decides: then b - o is exact and nonzero on every axis with opposite signs
+// at the two faces, so every axis gives one negative and one positive slab
+// value (no NaN, even with 1/d infinite), near stays 0 and far > 0, and
+// 0 <= m because every MinT is positive.  So the test is
+// near <= far && near <= m, and a deferred child's near (0 for an origin
+// inside it) is tested at pop time as near <= MinT.  The per-axis swap keeps
+// the reference's selections (with a NaN slab value, from 0 * inf, it
+// decides which value goes on).  The folds are fmaxf/fminf, which the
+// compiler merges into three-input max/min: the reference's
+// `near = tn > near ? tn : near` keeps near when tn is NaN, and so does fmaxf
+// because near starts at 0 and is never NaN (likewise far from +inf).  The
+// two can differ only in the sign of a zero result, and near and far are only
+// ever compared, where -0 == +0.
 __device__ __forceinline__ void slab(float b0, float b1, float b2, float b3, float b4, float b5,
-                                     const Ray &r, bool &inside, float &near, float &far) {
-   inside = r.ox > b0 && r.ox < b3 && r.oy > b1 && r.oy < b4 && r.oz > b2 && r.oz < b5;
+                                     const Ray &r, float &near, float &far) {
    near = 0.f;
    far = kInf;
    float tn, tf, sw;

@@ -61,7 +61,7 @@ __device__ __forceinline__ bool pop_next(Trav &s, const Stack<kLds> &st) {
       uint32_t r, c;
       float nr;
       st.get(s.sp, r, nr, c);
-      if (nr < 0.f || nr <= s.min_t) {
+      if (nr <= s.min_t) {
          s.bb_hits++;
          s.cur = r;
          s.cnt = c;
@@ -83,10 +83,9 @@ __device__ __forceinline__ void begin(Trav &s, const float *root_box, uint32_t r
    s.sp = 0;
    s.bb_hits = s.bb_misses = s.tris = 0;
    s.hu = s.hv = s.hw = 0.f;
-   bool inside;
    float near, far;
-   slab(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], s.r, inside, near, far);
-   if (inside || (near <= far && near <= s.min_t)) {
+   slab(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], s.r, near, far);
+   if (near <= far && near <= s.min_t) {
       s.bb_hits++;
       s.cur = root_ref;
       s.cnt = root_count;
@@ -182,21 +181,20 @@ enum : int { kStepDone = 0, kStepFirst = 1, kStepOther = 2 };
 // first child, kStepDone when the traversal has finished, else kStepOther.
 template <int kLds>
 __device__ __forceinline__ int interior_step(Trav &s, const float *q, const Stack<kLds> &st) {
-   bool in0, in1;
    float n0, f0, n1, f1;
-   slab(q[0], q[1], q[2], q[3], q[4], q[5], s.r, in0, n0, f0);
-   slab(q[6], q[7], q[8], q[9], q[10], q[11], s.r, in1, n1, f1);
+   slab(q[0], q[1], q[2], q[3], q[4], q[5], s.r, n0, f0);
+   slab(q[6], q[7], q[8], q[9], q[10], q[11], s.r, n1, f1);
    const uint32_t mref0 = __float_as_uint(q[12]), mref1 = __float_as_uint(q[13]);
    const uint32_t mcounts = __float_as_uint(q[14]), maxis = __float_as_uint(q[15]);
-   const bool h0 = in0 || (n0 <= f0 && n0 <= s.min_t);
-   const bool h1 = in1 || (n1 <= f1 && n1 <= s.min_t);
+   const bool h0 = n0 <= f0 && n0 <= s.min_t;
+   const bool h1 = n1 <= f1 && n1 <= s.min_t;
    // bvh.cpp:510-517: with neg_dir[axis] the high child (child 1) is popped first.
    const bool high_first = (s.r.nk >> maxis) & 1u;
    const bool fh = high_first ? h1 : h0, sh = high_first ? h0 : h1;
    const uint32_t c0 = mcounts & 0xFFFFu, c1 = mcounts >> 16;
    const uint32_t fref = high_first ? mref1 : mref0, sref = high_first ? mref0 : mref1;
    const uint32_t fcnt = high_first ? c1 : c0, scnt = high_first ? c0 : c1;
-   const float snear = high_first ? (in0 ? -1.f : n0) : (in1 ? -1.f : n1);
+   const float snear = high_first ? n0 : n1;
    if (fh) {
       s.bb_hits++;
       if (sh) {
