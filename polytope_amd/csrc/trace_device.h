@@ -124,10 +124,13 @@ __device__ __forceinline__ void slab(float b0, float b1, float b2, float b3, flo
    near = fmaxf(near, tn); far = fminf(far, tf);
 }
 
-__device__ __forceinline__ float max_component(float x, float y, float z) {   // mesh.cpp:149-155
-   if (x >= y && x >= z) return x;
-   else if (y >= x && y >= z) return y;
-   return z;
+// MaxComponent (mesh.cpp:149-155) of three absolute values, as a three-input
+// max.  The reference's comparison chain returns the same value whenever no
+// input is NaN.  Its results feed only deltaT, which matters only through
+// `t <= deltaT`, and a NaN input (a NaN coordinate, U, V or W) makes det or T,
+// and so t, NaN: the comparison is then false whatever deltaT is.
+__device__ __forceinline__ float max_component(float x, float y, float z) {
+   return fmaxf(fmaxf(x, y), z);
 }
 
 // Mesh::intersect test part for one face (mesh.cpp:196-340).  On acceptance
@@ -164,7 +167,11 @@ __device__ __forceinline__ bool test_triangle(const float *v, const Ray &r, floa
       const double ByAx = (double)p1y * (double)p0x;
       W = (float)(ByAx - BxAy);
    }
-   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+   // mesh.cpp:272: (U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0).
+   // fminf/fmaxf skip a NaN operand (and give NaN only when all three are), so
+   // the smallest and largest of the three decide exactly what the two
+   // disjunctions decide, zeros of either sign included.
+   if (fminf(fminf(U, V), W) < 0.0f && fmaxf(fmaxf(U, V), W) > 0.0f) return false;
    const float det = U + V + W;
    if (det == 0.0f) return false;
    p0z *= r.Sz; p1z *= r.Sz; p2z *= r.Sz;
