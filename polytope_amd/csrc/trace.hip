@@ -85,8 +85,13 @@ struct RayQuery {
    }
 };
 
+// The hits-only node-record form is held to at most 96 VGPRs, five waves per
+// SIMD (with a 12- or 8-entry LDS stack five blocks fit a CU's LDS too): four
+// blocks per CU ran 2,154 against 2,307-2,326 Mrays/s with five on 2^21
+// incoherent rays.  The counters/frames and window forms need more registers
+// and keep four waves rather than spill.
 template <int kLds, bool kExtra, bool kWin>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kExtra || kWin ? 4 : 5, 8)))
 k_closest_hit(DevScene sc, const float *__restrict__ rays, uint32_t n, ptx_hit *__restrict__ hits,
               ptx_ray_counters *counters, ptx_hit_frame *frames, uint32_t *work, uint4 *spill, uint32_t refill_min,
               uint32_t leaf_min, const Node64 *win, uint32_t root_cnt) {

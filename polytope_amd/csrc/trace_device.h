@@ -287,7 +287,7 @@ __device__ __forceinline__ void hit_frame(const float *__restrict__ v, const flo
 //     use, so its result is the reference's;
 //   * the other child is popped only after the first child's subtree; its slab
 //     interval is kept on the stack and re-checked at pop time against the
-//     MinT of that moment (inside || near <= MinT).  A child that misses now
+//     MinT of that moment (near <= MinT).  A child that misses now
 //     would miss later too (MinT only decreases), so it is not pushed.
 // Leaves test their triangles in leaf order, as the reference does.  The
 // counters reproduce the reference's num_bb_hits / num_bb_misses /
