@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include "launch.h"
+#include "divmagic.h"
 #include "traverse.h"
 
 #pragma clang fp contract(off)
@@ -1000,8 +1001,15 @@ struct LdsChunk {
    uint32_t base, npix, s0, width;
    uint32_t seed, max_depth, scene, flags;
    const uint32_t *primary;
-   const void *pad2;
+   DivU32 by_npix;            // local slot -> (sample, pixel) without a run-time division
 };
+
+// (sample offset, pixel) of a chunk's local slot: local = sample offset * npix + pixel.
+__device__ __forceinline__ void chunk_split(const LdsChunk &c, uint32_t local, uint32_t &sample_off,
+                                            uint32_t &pixel) {
+   sample_off = div_u32(local, c.by_npix);
+   pixel = local - sample_off * c.npix;
+}
 
 __device__ __forceinline__ uint32_t lds_find_chunk(const LdsChunk *c, uint32_t n, uint32_t slot) {
    uint32_t lo = 0, hi = n;
@@ -1057,7 +1065,7 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
       LdsChunk l;
       l.base = c.base; l.npix = c.npix; l.s0 = c.s0; l.width = J.width;
       l.seed = J.seed; l.max_depth = J.max_depth; l.scene = J.scene; l.flags = J.flags;
-      l.primary = J.primary; l.pad2 = nullptr;
+      l.primary = J.primary; l.by_npix = div_u32_setup(c.npix);
       sch[i] = l;
    }
    for (uint32_t i = threadIdx.x; i < nscenes; i += blockDim.x) ssc[i] = scenes[i];
@@ -1139,7 +1147,8 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          const uint32_t cf = lds_find_chunk(sch, nchunks, slot);
          mode = kPPrim | (cf << kPChunkShift);
          const LdsChunk &c = sch[cf];
-         const uint32_t pixel = (slot - c.base) % c.npix;
+         uint32_t sample_off, pixel;
+         chunk_split(c, slot - c.base, sample_off, pixel);
          const PTX_GLOBAL f32x4u *pr = as_global(reinterpret_cast<const f32x4u *>(c.primary + 16 * (size_t)pixel));
          const f32x4u q0 = pr[0], q1 = pr[1], q2 = pr[2], q3 = pr[3];
          f.q[0] = q0.x; f.q[1] = q0.y; f.q[2] = q0.z; f.q[3] = q0.w;
@@ -1255,9 +1264,9 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
          uint32_t scene = 0;                      // index into the round's scene table
          if (m == kPPrim) {
             const LdsChunk &c = sch[lw_chunk(mode)];
-            const uint32_t local = slot - c.base;
-            sample = c.s0 + local / c.npix;
-            pixel = local % c.npix;
+            uint32_t sample_off;
+            chunk_split(c, slot - c.base, sample_off, pixel);
+            sample = c.s0 + sample_off;
             const uint32_t w0 = __float_as_uint(f.q[0]);
             if (kNee) {
                direct[0] = direct[1] = direct[2] = 0.f;
@@ -1282,8 +1291,10 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
                   if (kGlossy) s.cnt = mw[5];      // glossy roughness
                }
                src[0] = src[1] = src[2] = 1.f;
-               const uint32_t x = pixel % c.width, y = pixel / c.width;
-               const uint32_t dbg = (x == 128 && y == 128) ? kPDbg : 0u;
+               // pixel (128, 128) (mesh.cpp:192-194): x == 128 needs width > 128, and
+               // then y * width + x == 128 * width + 128 holds for that pixel only
+               const uint32_t dbg =
+                     c.width > 128u && (uint64_t)pixel == 128ull * c.width + 128ull ? kPDbg : 0u;
                nb = (mode & ~kPModeMask) | dbg | (1u << kPBounceShift);
                b = 0u;
                seed = c.seed;
@@ -1364,9 +1375,9 @@ k_paths(const Chunk *chunks, uint32_t nchunks, uint32_t nslots, const JobDev *jo
                f.q[7] = fr.nx; f.q[8] = fr.ny; f.q[9] = fr.nz;
                f.q[10] = fr.t2x; f.q[11] = fr.t2y; f.q[12] = fr.t2z;
                f.q[13] = lin[0]; f.q[14] = lin[1]; f.q[15] = lin[2];
-               const uint32_t local = slot - c.base;
-               sample = c.s0 + local / c.npix;
-               pixel = local % c.npix;
+               uint32_t sample_off;
+               chunk_split(c, slot - c.base, sample_off, pixel);
+               sample = c.s0 + sample_off;
                b = bi;
                seed = c.seed;
                nb = (mode & ~kPModeMask) + (1u << kPBounceShift);
