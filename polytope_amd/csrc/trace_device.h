@@ -85,7 +85,7 @@ __device__ __forceinline__ void make_ray(Ray &r, float ox, float oy, float oz, f
    const float pdx = pick3(kx, dx, dy, dz), pdy = pick3(ky, dx, dy, dz), pdz = pick3(kz, dx, dy, dz);
    r.Sx = -pdx / pdz;                                          // mesh.cpp:234-236
    r.Sy = -pdy / pdz;
-   r.Sz = 1.f / pdz;
+   r.Sz = pick3(kz, r.ix, r.iy, r.iz);                         // 1 / pdz, the same division
 }
 
 // BoundingBox::Hits (BoundingBox.cpp:12-85) as the slab interval [near, far]
