@@ -280,7 +280,8 @@ typedef struct ptx_tuning {
    uint32_t paths_per_wave;  /* path slots per round                                  */
    uint32_t block_size;      /* threads per block of the traversal kernel (256)       */
    uint32_t lds_stack;       /* LDS stack entries per lane: 8, 12, 16 or 24           */
-   uint32_t refill_lanes;    /* idle lanes that trigger a wave's ray refill (1..64)   */
+   uint32_t refill_lanes;    /* ray queries: idle lanes that trigger a wave's ray     */
+                             /* refill (1..64; 0 = 4)                                */
    uint32_t blocks_per_cu;   /* persistent blocks per CU; 0 = occupancy limit         */
    uint32_t render_mode;     /* PTX_RENDER_*: AUTO and REGEN = path regeneration     */
                              /* (rounds of at most 64 chunks; a batch of more than  */

@@ -150,7 +150,7 @@ int run_closest_hit(ptx_scene *s, const float *d_rays, uint32_t n, ptx_hit *d_hi
    PTX_HIP_TRY(pool_malloc(dev, &scratch, closest_hit_scratch_bytes(s, grid, lds), stream));
    uint32_t *work = static_cast<uint32_t *>(scratch);
    uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(scratch) + 256);
-   const uint32_t refill = t.refill_lanes ? std::min(64u, t.refill_lanes) : 1u;
+   const uint32_t refill = refill_lanes_of(t);
    const uint32_t leaf = leaf_lanes_of(t);
    const int rc = launch_closest_hit(s->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill, leaf,
                                      win, s->root_axis, stream);
@@ -466,7 +466,7 @@ int ptx_intersect_closest_device_noalloc(ptx_scene *scene, const float *d_rays, 
    PTX_HIP_TRY(hipSetDevice(dev->ordinal));
    uint32_t *work = static_cast<uint32_t *>(dev->noalloc_scratch);
    uint4 *spill = reinterpret_cast<uint4 *>(static_cast<char *>(dev->noalloc_scratch) + 256);
-   const uint32_t refill = t.refill_lanes ? std::min(64u, t.refill_lanes) : 1u;
+   const uint32_t refill = refill_lanes_of(t);
    // the work counters are cleared by a kernel at the head of the launch (no memset)
    if (launch_closest_hit(scene->d, d_rays, n, d_hits, d_counters, d_frames, work, spill, grid, lds, refill,
                           leaf_lanes_of(t), win, scene->root_axis, static_cast<hipStream_t>(stream)) != 0) {
@@ -506,7 +506,7 @@ int ptx_get_tuning(ptx_device *dev, ptx_tuning *t) {
    *t = tuning_of(dev);
    if (!t->lds_stack) t->lds_stack = lds_depth_of(*t);
    if (!t->block_size) t->block_size = 256;
-   if (!t->refill_lanes) t->refill_lanes = 1;
+   if (!t->refill_lanes) t->refill_lanes = kDefaultRefillLanes;
    // leaf_lanes stays 0 when unset: each kernel family then uses its own default
    if (!t->shade_lanes) t->shade_lanes = RenderEngine::kDefaultShadeLanes;
    return PTX_OK;

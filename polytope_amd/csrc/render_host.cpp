@@ -677,7 +677,7 @@ extern "C" int ptx_autotune(ptx_device *dev, ptx_scene *scene, ptx_tuning *chose
    {
       std::lock_guard<std::mutex> lock(dev->mu);
       for (uint32_t lds : {8u, 12u, 16u})
-         for (uint32_t refill : {1u, 8u, 32u}) {
+         for (uint32_t refill : {1u, 4u, 8u, 32u}) {
             trial.lds_stack = lds;
             trial.refill_lanes = refill;
             float ms_min = 1e30f;
